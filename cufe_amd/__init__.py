@@ -1,0 +1,203 @@
+"""cufe_amd -- Python bindings of libcufe_ipfe.so, the MI355X (gfx950) RLWE-IPFE
+("rlwe_sife") library.
+
+The product is the HIP library behind the C ABI in include/cufe_ipfe.h; this
+module only marshals numpy buffers to it.  There is no CPU fallback: if the
+library is missing, importing a function raises.
+
+Names mirror the reference's entry points (src/rlwe_sife.h, src/rlwe_sife_gpu.cu):
+setup / encrypt / keygen / decrypt, sample_polya, round_extract, svm_predict.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+N = 4096
+NMODULI = 3
+L = 785
+P = 50241
+Q = (16760833, 2147352577, 2130706433)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcufe_ipfe.so")
+_lib = None
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class CufeError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CufeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lb = C.CDLL(LIB_PATH)
+    sig = {
+        "cufe_last_error": (C.c_char_p, []),
+        "cufe_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+        "cufe_ntt_forward": (C.c_int, [_u32p, C.c_int]),
+        "cufe_ntt_inverse": (C.c_int, [_u32p, C.c_int]),
+        "cufe_gauss_sample": (C.c_int, [_u8p, _u16p, C.c_int, C.c_int, C.c_int, _u32p]),
+        "cufe_aes256ctr": (C.c_int, [_u8p, C.c_uint16, C.c_uint32, C.c_uint32, _u8p]),
+        "cufe_crt_round": (C.c_int, [_u32p, C.c_int, C.c_int, _u32p]),
+        "cufe_sample_polya": (C.c_int, [_u8p, _u32p]),
+        "cufe_round_extract2": (C.c_double, [_u32p]),
+        "cufe_sife_setup": (C.c_int, [_u32p, _u32p, _u8p, _u8p, C.c_int]),
+        "cufe_sife_encrypt": (C.c_int, [_u32p, _u32p, _u32p, _u8p, _u8p, C.c_int, C.c_int, C.c_int]),
+        "cufe_sife_keygen": (C.c_int, [_u32p, _u32p, _u32p, C.c_int, C.c_int]),
+        "cufe_sife_decrypt": (C.c_int, [_u32p, _u32p, _u32p, _u32p, C.c_int, C.c_int]),
+        "cufe_time_ntt": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "cufe_time_gauss": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "rlwe_sife_setup_gpu": (None, [_u32p, _u32p, _u8p, _u8p, C.POINTER(C.c_float)]),
+        "rlwe_sife_encrypt_gpu": (None, [_u32p, _u32p, _u32p, _u8p, _u8p, C.c_int, C.POINTER(C.c_float)]),
+        "rlwe_sife_keygen_gpu": (None, [_u32p, _u32p, _u32p, C.c_int, C.POINTER(C.c_float)]),
+        "rlwe_sife_decrypt_gmp_gpu": (None, [_u32p, _u32p, _u32p, _u32p, C.c_int, C.POINTER(C.c_float)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lb, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lb
+    return lb
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().cufe_last_error()
+        raise CufeError(f"{what}: {msg.decode() if msg else 'unknown error'}")
+
+
+def _seed(s) -> np.ndarray:
+    a = np.frombuffer(bytes(s), dtype=np.uint8).copy()
+    if a.size != 32:
+        raise ValueError("seeds are 32 bytes")
+    return a
+
+
+def _u32(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    if shape is not None and a.shape != tuple(shape):
+        a = a.reshape(shape)
+    return a
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(256)
+    cu = C.c_int()
+    hbm = C.c_uint64()
+    _check(lib().cufe_device_info(name, 256, C.byref(cu), C.byref(hbm)), "device_info")
+    return {"name": name.value.decode(), "cu": cu.value, "hbm_bytes": hbm.value}
+
+
+# ---- single operations -------------------------------------------------------
+def ntt_forward(polys) -> np.ndarray:
+    """CT_forward (src/ntt.c:38-67) on [npoly][3][N] residues."""
+    a = _u32(polys).copy()
+    npoly = a.size // (NMODULI * N)
+    _check(lib().cufe_ntt_forward(a, npoly), "ntt_forward")
+    return a
+
+
+def ntt_inverse(polys) -> np.ndarray:
+    """GS_reverse (src/ntt.c:70-114) including N^-1."""
+    a = _u32(polys).copy()
+    npoly = a.size // (NMODULI * N)
+    _check(lib().cufe_ntt_inverse(a, npoly), "ntt_inverse")
+    return a
+
+
+def gauss_sample(keys, nonces, which: int, ncalls: int) -> np.ndarray:
+    """ncalls consecutive gaussian_sampler_S{which} calls on each stream ->
+    [nstreams][ncalls][3][N]."""
+    k = np.ascontiguousarray(np.frombuffer(b"".join(bytes(x) for x in keys), dtype=np.uint8))
+    n = np.ascontiguousarray(nonces, dtype=np.uint16)
+    out = np.empty((len(n), ncalls, NMODULI, N), dtype=np.uint32)
+    _check(lib().cufe_gauss_sample(k, n, len(n), which, ncalls, out), "gauss_sample")
+    return out
+
+
+def aes256ctr(key, nonce: int, ctr0: int, nblocks: int) -> bytes:
+    out = np.empty(nblocks * 16, dtype=np.uint8)
+    _check(lib().cufe_aes256ctr(_seed(key), nonce, ctr0, nblocks, out), "aes256ctr")
+    return out.tobytes()
+
+
+def sample_polya(seed) -> np.ndarray:
+    a = np.empty((NMODULI, N), dtype=np.uint32)
+    _check(lib().cufe_sample_polya(_seed(seed), a), "sample_polya")
+    return a
+
+
+def crt_round(d, coef: int = 0) -> np.ndarray:
+    d = _u32(d)
+    nvec = d.size // (NMODULI * N)
+    r = np.empty(nvec, dtype=np.uint32)
+    _check(lib().cufe_crt_round(d, nvec, coef, r), "crt_round")
+    return r
+
+
+def round_extract2(d_y) -> float:
+    return lib().cufe_round_extract2(_u32(d_y))
+
+
+# ---- scheme at vector length l -------------------------------------------------
+def setup(a_coeff, seed2, seed3, l: int = L):
+    """rlwe_sife_setup_gpu at length l: returns (mpk [l+1][3][N], msk [l][3][N])."""
+    mpk = np.zeros((l + 1, NMODULI, N), dtype=np.uint32)
+    mpk[l] = _u32(a_coeff, (NMODULI, N))
+    msk = np.zeros((max(l, 1), NMODULI, N), dtype=np.uint32)
+    _check(lib().cufe_sife_setup(mpk, msk, _seed(seed2), _seed(seed3), l), "setup")
+    return mpk, msk[:l]
+
+
+def encrypt(m, mpk, seed2, seed3, nonce0: int = 0):
+    """Encrypt rows m [repeat][l] -> c [repeat][l+1][3][N] (message b uses nonce nonce0+b)."""
+    m = _u32(np.atleast_2d(m))
+    repeat, l = m.shape
+    mpk = _u32(mpk)
+    assert mpk.shape == (l + 1, NMODULI, N)
+    c = np.empty((repeat, l + 1, NMODULI, N), dtype=np.uint32)
+    _check(lib().cufe_sife_encrypt(m if m.size else np.zeros(1, np.uint32), mpk, c, _seed(seed2), _seed(seed3),
+                                   repeat, l, nonce0), "encrypt")
+    return c
+
+
+def keygen(y, msk):
+    y = _u32(np.atleast_2d(y))
+    repeat, l = y.shape
+    sk = np.empty((repeat, NMODULI, N), dtype=np.uint32)
+    _check(lib().cufe_sife_keygen(y, _u32(msk), sk, repeat, l), "keygen")
+    return sk
+
+
+def decrypt(c, y, sk):
+    """One ciphertext c [l+1][3][N] against keys sk [repeat][3][N] -> d_y [repeat][3][N]."""
+    y = _u32(np.atleast_2d(y))
+    repeat, l = y.shape
+    d = np.empty((repeat, NMODULI, N), dtype=np.uint32)
+    _check(lib().cufe_sife_decrypt(_u32(c), y, _u32(sk), d, repeat, l), "decrypt")
+    return d
+
+
+def time_ntt(npoly: int, inverse: bool, iters: int) -> float:
+    ms = C.c_float()
+    _check(lib().cufe_time_ntt(npoly, int(inverse), iters, C.byref(ms)), "time_ntt")
+    return ms.value
+
+
+def time_gauss(nstreams: int, which: int, iters: int) -> float:
+    ms = C.c_float()
+    _check(lib().cufe_time_gauss(nstreams, which, iters, C.byref(ms)), "time_gauss")
+    return ms.value

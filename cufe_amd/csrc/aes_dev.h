@@ -1,0 +1,55 @@
+// aes_dev.h -- AES-256 encryption of CTR counter blocks with T-tables in LDS.
+//
+// Counter block of stream (key, nonce) at counter t is BE16(nonce) || 0^10 ||
+// BE32(t) (the block layout src/aes256ctr.c:27-40 produces).  Round keys are
+// the FIPS-197 big-endian words; the four T-tables (1 KiB each) are staged in
+// LDS by the calling workgroup.  The output block is returned as four
+// little-endian words, i.e. word w holds bytes 4w..4w+3 of the keystream, which
+// is how every consumer reads it (LE32 / LE64 loads).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ipfe {
+
+constexpr int AES_TABLE_WORDS = 4 * 256;
+
+__device__ __forceinline__ void aes_stage_tables(uint32_t *lds_te, const uint32_t *__restrict__ te_global)
+{
+    for (int i = threadIdx.x; i < AES_TABLE_WORDS; i += blockDim.x) lds_te[i] = te_global[i];
+}
+
+// rk: 60 round-key words, uniform across the workgroup (scalar loads).
+__device__ __forceinline__ void aes256_ctr_block(const uint32_t *__restrict__ rk, const uint32_t *te, uint32_t nonce,
+                                                 uint32_t ctr, uint32_t (&o)[4])
+{
+    const uint32_t *T0 = te, *T1 = te + 256, *T2 = te + 512, *T3 = te + 768;
+    uint32_t s0 = (nonce << 16) ^ rk[0];
+    uint32_t s1 = rk[1];
+    uint32_t s2 = rk[2];
+    uint32_t s3 = ctr ^ rk[3];
+#pragma unroll
+    for (int r = 1; r < 14; ++r) {
+        uint32_t t0 = T0[s0 >> 24] ^ T1[(s1 >> 16) & 255] ^ T2[(s2 >> 8) & 255] ^ T3[s3 & 255] ^ rk[4 * r];
+        uint32_t t1 = T0[s1 >> 24] ^ T1[(s2 >> 16) & 255] ^ T2[(s3 >> 8) & 255] ^ T3[s0 & 255] ^ rk[4 * r + 1];
+        uint32_t t2 = T0[s2 >> 24] ^ T1[(s3 >> 16) & 255] ^ T2[(s0 >> 8) & 255] ^ T3[s1 & 255] ^ rk[4 * r + 2];
+        uint32_t t3 = T0[s3 >> 24] ^ T1[(s0 >> 16) & 255] ^ T2[(s1 >> 8) & 255] ^ T3[s2 & 255] ^ rk[4 * r + 3];
+        s0 = t0; s1 = t1; s2 = t2; s3 = t3;
+    }
+    // Final round (SubBytes, ShiftRows, AddRoundKey): the S-box byte sits in
+    // byte 3 of T2, byte 2 of T3, byte 1 of T0 and byte 0 of T1.
+    uint32_t f0 = (T2[s0 >> 24] & 0xff000000u) ^ (T3[(s1 >> 16) & 255] & 0x00ff0000u) ^
+                  (T0[(s2 >> 8) & 255] & 0x0000ff00u) ^ (T1[s3 & 255] & 0x000000ffu) ^ rk[56];
+    uint32_t f1 = (T2[s1 >> 24] & 0xff000000u) ^ (T3[(s2 >> 16) & 255] & 0x00ff0000u) ^
+                  (T0[(s3 >> 8) & 255] & 0x0000ff00u) ^ (T1[s0 & 255] & 0x000000ffu) ^ rk[57];
+    uint32_t f2 = (T2[s2 >> 24] & 0xff000000u) ^ (T3[(s3 >> 16) & 255] & 0x00ff0000u) ^
+                  (T0[(s0 >> 8) & 255] & 0x0000ff00u) ^ (T1[s1 & 255] & 0x000000ffu) ^ rk[58];
+    uint32_t f3 = (T2[s3 >> 24] & 0xff000000u) ^ (T3[(s0 >> 16) & 255] & 0x00ff0000u) ^
+                  (T0[(s1 >> 8) & 255] & 0x0000ff00u) ^ (T1[s2 & 255] & 0x000000ffu) ^ rk[59];
+    o[0] = __builtin_bswap32(f0);
+    o[1] = __builtin_bswap32(f1);
+    o[2] = __builtin_bswap32(f2);
+    o[3] = __builtin_bswap32(f3);
+}
+
+}  // namespace ipfe

@@ -1,0 +1,370 @@
+// k_sampler.hip -- discrete Gaussian sampling on gfx950, bit-exact with
+// gaussian_sampler_S1_ori / S2_ori / S3 (src/gauss.c:705-785, :915-993,
+// :1086-1163) on the AES-256-CTR stream of src/aes256ctr.c.
+//
+// The reference draws sample after sample from one sequential stream.  Each
+// 384-byte refill ("chunk", 24 AES blocks) yields 8 candidates; a call takes
+// accepted candidates in order until it has 4096, drops the rest of its last
+// chunk, and the next call on the stream starts on the next chunk.  Here that
+// becomes three data-parallel passes:
+//   1. candidates: every chunk of a speculative window, in parallel: AES (LDS
+//      T-tables), uniform/CDT/Bernoulli per candidate, wave ballot -> per-chunk
+//      accept count and the accepted values compacted in order;
+//   2. walk: one wave per stream turns counts into call boundaries with
+//      1024-chunk windows and a wave prefix scan, and labels every chunk with
+//      (call, first output position);
+//   3. scatter: every accepted candidate of a used chunk writes its three
+//      residues to its call's polynomial.
+#include "aes_dev.h"
+#include "kernels.h"
+
+namespace ipfe {
+
+// Sampler constants (src/gauss.c:11-45): k per width, the Barrett factor
+// floor(2^32/k), the uniform acceptance bound k*floor(2^32/k) and -1/k^2.
+__constant__ uint64_t c_K[4] = {1ull, 266ull, 304214978ull, 608429953ull};
+__constant__ uint64_t c_BF[4] = {0ull, (1ull << 32) / 266ull, (1ull << 32) / 304214978ull, (1ull << 32) / 608429953ull};
+__constant__ uint64_t c_UQ[4] = {0ull, 266ull * ((1ull << 32) / 266ull), 304214978ull * ((1ull << 32) / 304214978ull),
+                                 608429953ull * ((1ull << 32) / 608429953ull)};
+__constant__ double c_K2INV[4] = {0.0, -1.0 / (double)(long long)(266ll * 266ll),
+                                  -1.0 / (double)(long long)(304214978ll * 304214978ll),
+                                  -1.0 / (double)(long long)(608429953ll * 608429953ll)};
+
+// CDT rows (low word, high word), src/gauss.c:62-72.
+__constant__ uint64_t c_CDT[9][2] = {{2200310400551559144ull, 3327841033070651387ull},
+                                     {7912151619254726620ull, 380075531178589176ull},
+                                     {5167367257772081627ull, 11604843442081400ull},
+                                     {5081592746475748971ull, 90134450315532ull},
+                                     {6522074513864805092ull, 175786317361ull},
+                                     {2579734681240182346ull, 85801740ull},
+                                     {8175784047440310133ull, 10472ull},
+                                     {2947787991558061753ull, 0ull},
+                                     {22489665999543ull, 0ull}};
+
+// Coefficients of the 2^a polynomial, src/gauss.c:82-103 (IEEE-754 bit patterns).
+__constant__ uint64_t c_EXP[10] = {0x3e833b70ffa2c5d4ull, 0x3eb4a480fda7e6e1ull, 0x3ef01b254493363full,
+                                   0x3f242e0e0aa273ccull, 0x3f55d8a2334ed31bull, 0x3f83b2aa56db0f1aull,
+                                   0x3fac6b08e11fc57eull, 0x3fcebfbdff556072ull, 0x3fe62e42fefa7fe6ull,
+                                   0x3ff0000000000000ull};
+
+extern const uint32_t *g_aes_te;  // device T-tables, set by the context
+
+constexpr int CAND_THREADS = 256;
+constexpr int TILE_CHUNKS = 64;
+constexpr int CHUNK_WORDS = 96;
+
+__device__ __forceinline__ int find_stream(const long long *prefix, int n, long long key)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= key) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Bernoulli decision (src/gauss.c:204-264 and the S2/S3 copies): the
+// reference's exact sequence of IEEE double operations and integer bit tricks.
+// rb points at the 36 Bernoulli bytes of the candidate's group.
+__device__ __forceinline__ bool bernoulli_bit(uint64_t b, double k2inv, const uint32_t *rb, int ln)
+{
+    double vx = __longlong_as_double((long long)(b | 0x4330000000000000ull)) - 4503599627370496.0;
+    vx = __dmul_rn(vx, k2inv);
+    double fl = floor(vx);
+    double sh = __dadd_rn(fl, 6755399441055744.0);
+    uint64_t t = ((uint64_t)__double_as_longlong(sh) - 0x4338000000000000ull) << 52;
+    double a = __dsub_rn(vx, fl);
+    double s = __fma_rn(__longlong_as_double((long long)c_EXP[0]), a, __longlong_as_double((long long)c_EXP[1]));
+#pragma unroll
+    for (int i = 2; i < 10; ++i) s = __fma_rn(s, a, __longlong_as_double((long long)c_EXP[i]));
+    uint64_t res = t + (uint64_t)__double_as_longlong(s);
+    uint64_t res_mant = (res & ((1ull << 52) - 1)) | (1ull << 52);
+    uint64_t e = (res >> 52) + (uint64_t)(long long)-1003;
+    uint64_t res_exp = (e < 64) ? (1ull << e) : 0ull;
+    uint64_t rm = ((uint64_t)rb[2 * ln + 1] << 32) | rb[2 * ln];
+    uint64_t r_exp = (rm >> 53) | ((uint64_t)((rb[8] >> (8 * ln)) & 255u) << 11);
+    uint64_t r_mant = rm & ((1ull << 53) - 1);
+    uint64_t bits = (r_mant - res_mant) & (r_exp - res_exp);
+    return (bits >> 63) != 0 || res == 0x3FF0000000000000ull;
+}
+
+__global__ __launch_bounds__(CAND_THREADS) void k_gauss_candidates(const GaussStream *__restrict__ streams, int nstreams,
+                                                                    const long long *__restrict__ tile_prefix,
+                                                                    const uint32_t *__restrict__ te_global,
+                                                                    uint8_t *__restrict__ cnt, uint64_t *__restrict__ vals,
+                                                                    int *__restrict__ status)
+{
+    __shared__ uint32_t te[AES_TABLE_WORDS];
+    __shared__ uint32_t ks[TILE_CHUNKS * CHUNK_WORDS];
+    const int si = __builtin_amdgcn_readfirstlane(find_stream(tile_prefix, nstreams, (long long)blockIdx.x));
+    const GaussStream *S = streams + si;
+    const long long chunk0 = ((long long)blockIdx.x - tile_prefix[si]) * TILE_CHUNKS;
+    const int nch = (int)min((long long)TILE_CHUNKS, (long long)S->nchunks - chunk0);
+    const int which = S->which;
+    const uint32_t nonce = S->nonce;
+
+    aes_stage_tables(te, te_global);
+    __syncthreads();
+    for (int g = threadIdx.x; g < nch * 24; g += CAND_THREADS) {
+        const int k = g / 24, blk = g - 24 * k;
+        const uint32_t ctr = (uint32_t)((chunk0 + k) * 24 + blk);
+        uint32_t o[4];
+        aes256_ctr_block(S->rk, te, nonce, ctr, o);
+        uint32_t *d = ks + k * CHUNK_WORDS + blk * 4;
+        d[0] = o[0]; d[1] = o[1]; d[2] = o[2]; d[3] = o[3];
+    }
+    __syncthreads();
+
+    const uint64_t K = c_K[which], BF = c_BF[which];
+    const uint32_t UQ = (uint32_t)c_UQ[which];
+    const double K2INV = c_K2INV[which];
+    const int lane = threadIdx.x & 63;
+    const int sh = lane & 56;
+
+    for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += CAND_THREADS) {
+        const int k = slot >> 3, c = slot & 7;
+        const bool valid = k < nch;
+        const uint32_t *ch = ks + k * CHUNK_WORDS;
+        // uniform_sampler (src/gauss.c:609-703): the c-th word below UQ from byte 200 on
+        const uint32_t w0 = ch[50 + c], w1 = ch[58 + c], w2 = ch[66 + c];
+        const uint64_t m0 = __ballot(w0 < UQ), m1 = __ballot(w1 < UQ), m2 = __ballot(w2 < UQ);
+        const uint32_t mask = (uint32_t)((m0 >> sh) & 0xffu) | ((uint32_t)((m1 >> sh) & 0xffu) << 8) |
+                              ((uint32_t)((m2 >> sh) & 0xffu) << 16);
+        int pos = c;
+        if ((mask & 0xffu) != 0xffu) {
+            int seen = 0;
+            pos = -1;
+            for (int u = 0; u < 46; ++u) {
+                const bool kept = (u < 24) ? ((mask >> u) & 1u) : (ch[50 + u] < UQ);
+                if (kept) {
+                    if (seen == c) { pos = u; break; }
+                    ++seen;
+                }
+            }
+            if (pos < 0) {
+                if (valid) atomicOr(status, 2);
+                pos = 0;
+            }
+        }
+        const uint64_t w = ch[50 + pos];
+        uint64_t y = w - ((((w * BF) >> 32) + 1) * K);
+        y = y + (y >> 63) * K;
+
+        // cdt_sampler (src/gauss.c:122-154)
+        const int g = c >> 2, ln = c & 3;
+        const uint32_t *gb = ch + 25 * g;
+        const uint64_t r1 = (((uint64_t)gb[2 * ln + 1] << 32) | gb[2 * ln]) & 0x7fffffffffffffffull;
+        const uint64_t r2 = (((uint64_t)gb[8 + 2 * ln + 1] << 32) | gb[8 + 2 * ln]) & 0x7fffffffffffffffull;
+        uint32_t xs = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) xs += (r2 < c_CDT[i][1]) | ((r2 == c_CDT[i][1]) & (r1 < c_CDT[i][0]));
+
+        const uint64_t xk = (uint64_t)xs * (uint64_t)(uint32_t)K;
+        const uint64_t z = xk + y;
+        const uint64_t bin = (uint64_t)(uint32_t)(z + xk) * (uint64_t)(uint32_t)y;
+        const bool bit = bernoulli_bit(bin, K2INV, gb + 16, ln);
+        const uint32_t sgn = (ch[70] >> c) & 1u;
+        const bool acc = bit && (z != 0 || sgn != 0);
+
+        const uint64_t am = __ballot(acc);
+        const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
+        if (valid) {
+            const long long gc = S->chunk_base + chunk0 + k;
+            if (c == 0) cnt[gc] = (uint8_t)__popc(gm);
+            if (acc) vals[gc * 8 + __popc(gm & ((1u << c) - 1u))] = z | ((uint64_t)sgn << 63);
+        }
+    }
+}
+
+// One wave per stream.  Window of 64 lanes x 16 chunks.
+__global__ __launch_bounds__(64) void k_gauss_walk(const GaussStream *__restrict__ streams, const uint8_t *__restrict__ cnt,
+                                                    int *__restrict__ pos, int *__restrict__ status)
+{
+    const GaussStream *S = streams + blockIdx.x;
+    const int lane = threadIdx.x;
+    const long long base = S->chunk_base;
+    const int nchunks = S->nchunks, ncalls = S->ncalls;
+    int c = 0, call = 0, filled = 0;
+    while (call < ncalls) {
+        if (c >= nchunks) {
+            if (lane == 0) status[blockIdx.x] = 1;
+            return;
+        }
+        uint32_t v[16];
+        uint32_t lsum = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int ci = c + 16 * lane + r;
+            v[r] = (ci < nchunks) ? cnt[base + ci] : 0u;
+            lsum += v[r];
+        }
+        // inclusive wave scan of lsum
+        uint32_t incl = lsum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        const uint32_t excl = incl - lsum;
+        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t need = 4096u - (uint32_t)filled;
+        const uint64_t hit = __ballot(incl >= need);
+        int e_chunk = -1;  // last chunk of this call, relative to c
+        if (hit) {
+            const int hl = __ffsll((unsigned long long)hit) - 1;
+            if (lane == hl) {
+                uint32_t run = excl;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    run += v[r];
+                    if (run >= need && e_chunk < 0) e_chunk = 16 * lane + r;
+                }
+            }
+            e_chunk = __shfl(e_chunk, hl, 64);
+        }
+        const int last = (e_chunk >= 0) ? e_chunk : 1023;
+        uint32_t run = excl;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int rel = 16 * lane + r;
+            const int ci = c + rel;
+            if (rel <= last && ci < nchunks) pos[base + ci] = (call << 13) | (int)((uint32_t)filled + run);
+            run += v[r];
+        }
+        if (e_chunk >= 0) {
+            ++call;
+            filled = 0;
+            c += e_chunk + 1;
+        } else {
+            filled += (int)total;
+            c += 1024;
+        }
+    }
+}
+
+__global__ void k_gauss_scatter(const GaussStream *__restrict__ streams, int nstreams,
+                                const long long *__restrict__ chunk_prefix, long long nchunks_total,
+                                const uint8_t *__restrict__ cnt, const uint64_t *__restrict__ vals,
+                                const int *__restrict__ pos)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gc = t >> 3;
+    const int slot = (int)(t & 7);
+    if (gc >= nchunks_total) return;
+    const int info = pos[gc];
+    if (info < 0) return;
+    const int n = cnt[gc];
+    const int call = info >> 13, off = info & 0x1fff;
+    if (slot >= n || off + slot >= IPFE_N) return;
+    const int si = find_stream(chunk_prefix, nstreams, gc);
+    const GaussStream *S = streams + si;
+    uint32_t *out = (call == 0) ? S->out0 : S->out1 + (long long)(call - 1) * S->out_stride;
+    const uint64_t v = vals[gc * 8 + slot];
+    const uint64_t z = v & 0x7fffffffffffffffull;
+    const bool neg = (v >> 63) != 0;
+    // output residues (src/gauss.c:773-779)
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        uint32_t m = mod_prime_exact(z, s);
+        if (neg) m = mod_prime_exact((uint32_t)(qmod(s) - m), s);
+        out[s * IPFE_N + off + slot] = m;
+    }
+}
+
+__global__ void k_fill_i32(int *p, int v, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// sample_polya (src/sample.c:56-128): coefficient i = LE64(keystream[8i..8i+8))
+// mod q_j, keystream of AES-256-CTR(seed, nonce 0) from counter 0.
+__global__ void k_sample_polya(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global,
+                               uint32_t *__restrict__ a)
+{
+    __shared__ uint32_t te[AES_TABLE_WORDS];
+    aes_stage_tables(te, te_global);
+    __syncthreads();
+    const uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x;  // 2048 blocks
+    if (blk >= IPFE_N / 2) return;
+    uint32_t o[4];
+    aes256_ctr_block(rk, te, 0u, blk, o);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint64_t v = ((uint64_t)o[2 * h + 1] << 32) | o[2 * h];
+        const uint32_t i = 2 * blk + h;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) a[s * IPFE_N + i] = (uint32_t)(v % qmod(s));
+    }
+}
+
+__global__ void k_aes_ctr(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global, uint32_t nonce,
+                          uint32_t ctr0, uint32_t nblocks, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t te[AES_TABLE_WORDS];
+    aes_stage_tables(te, te_global);
+    __syncthreads();
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    uint32_t o[4];
+    aes256_ctr_block(rk, te, nonce, ctr0 + b, o);
+    out[4 * b + 0] = o[0]; out[4 * b + 1] = o[1]; out[4 * b + 2] = o[2]; out[4 * b + 3] = o[3];
+}
+
+// ---------------------------------------------------------------------------
+int launch_gauss_candidates(const GaussStream *d_streams, int nstreams, const long long *d_tile_prefix,
+                            long long ntiles, uint8_t *cnt, uint64_t *vals, int *status, hipStream_t st)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_gauss_candidates, dim3((unsigned)ntiles), dim3(CAND_THREADS), 0, st, d_streams, nstreams,
+                       d_tile_prefix, g_aes_te, cnt, vals, status);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_gauss_walk(const GaussStream *d_streams, int nstreams, const uint8_t *cnt, int *pos, int *status,
+                      hipStream_t st)
+{
+    if (nstreams <= 0) return 0;
+    hipLaunchKernelGGL(k_gauss_walk, dim3(nstreams), dim3(64), 0, st, d_streams, cnt, pos, status);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_gauss_scatter(const GaussStream *d_streams, int nstreams, const long long *d_chunk_prefix,
+                         long long nchunks_total, const uint8_t *cnt, const uint64_t *vals, const int *pos,
+                         hipStream_t st)
+{
+    if (nchunks_total <= 0) return 0;
+    const long long nthreads = nchunks_total * 8;
+    hipLaunchKernelGGL(k_gauss_scatter, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, d_streams,
+                       nstreams, d_chunk_prefix, nchunks_total, cnt, vals, pos);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_fill_i32(int *p, int v, size_t n, hipStream_t st)
+{
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_sample_polya(const uint32_t *d_rk, uint32_t *a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sample_polya, dim3(IPFE_N / 2 / 256), dim3(256), 0, st, d_rk, g_aes_te, a);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_aes_ctr(const uint32_t *d_rk, uint32_t nonce, uint32_t ctr0, uint32_t nblocks, uint8_t *out, hipStream_t st)
+{
+    if (nblocks == 0) return 0;
+    hipLaunchKernelGGL(k_aes_ctr, dim3((nblocks + 255) / 256), dim3(256), 0, st, d_rk, g_aes_te, nonce, ctr0, nblocks,
+                       (uint32_t *)out);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace ipfe

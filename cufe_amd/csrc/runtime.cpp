@@ -1,0 +1,182 @@
+// runtime.cpp -- device state, error reporting and the Gaussian-sampler driver.
+#include "runtime.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+namespace ipfe {
+
+const uint32_t *g_aes_te = nullptr;
+
+static thread_local char t_err[1024];
+
+}  // namespace ipfe
+
+void ipfe_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ipfe::t_err, sizeof ipfe::t_err, fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *cufe_last_error(void) { return ipfe::t_err; }
+
+namespace ipfe {
+
+static Device *init_device()
+{
+    const HostTables &ht = host_tables();
+    static Device dev;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        ipfe_set_error("no HIP device available");
+        return nullptr;
+    }
+    if (hipMalloc((void **)&dev.tw_all, ht.tw_all.size() * 4) != hipSuccess ||
+        hipMalloc((void **)&dev.aes_te, ht.aes_te.size() * 4) != hipSuccess) {
+        ipfe_set_error("hipMalloc of constant tables failed");
+        return nullptr;
+    }
+    if (hipMemcpy(dev.tw_all, ht.tw_all.data(), ht.tw_all.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dev.aes_te, ht.aes_te.data(), ht.aes_te.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        ipfe_set_error("upload of constant tables failed");
+        return nullptr;
+    }
+    dev.ninv = ht.ninv;
+    dev.scale = ht.scale;
+    dev.barrett = ht.barrett;
+    dev.stream = nullptr;  // default stream
+    g_aes_te = dev.aes_te;
+    return &dev;
+}
+
+Device *device()
+{
+    static std::once_flag once;
+    static Device *d = nullptr;
+    static char err[1024];
+    std::call_once(once, [] {
+        d = init_device();
+        if (!d) snprintf(err, sizeof err, "%s", cufe_last_error());
+    });
+    if (!d) ipfe_set_error("device initialisation failed: %s", err);
+    return d;
+}
+
+int sync_stream(hipStream_t st)
+{
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncalls, uint32_t *out0, uint32_t *out1,
+                      long long out_stride)
+{
+    StreamReq r;
+    aes256_expand_key(key, r.rk);
+    r.nonce = nonce;
+    r.which = which;
+    r.ncalls = ncalls;
+    r.out0 = out0;
+    r.out1 = out1;
+    r.out_stride = out_stride;
+    return r;
+}
+
+// A call consumes 752.5 chunks on average (4096 samples at 8 x 0.6804 per
+// chunk) with a standard deviation of about 6.7 chunks; 770 per call plus 256
+// is > 40 sigma for any call count.  Level m doubles the margin m times.
+long long gauss_window(int ncalls, int m)
+{
+    long long base = (long long)ncalls * 770 + 256;
+    long long extra = (long long)ncalls * 17 + 256;
+    for (int i = 0; i < m; ++i) extra *= 2;
+    return base - ((long long)ncalls * 17 + 256) + extra;
+}
+
+static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t st, int *overflow)
+{
+    constexpr int kTile = 64;
+    std::vector<GaussStream> hs(n);
+    std::vector<long long> tile_prefix(n), chunk_prefix(n);
+    long long tiles = 0, chunks = 0;
+    for (int i = 0; i < n; ++i) {
+        GaussStream &g = hs[i];
+        memcpy(g.rk, reqs[i].rk, sizeof g.rk);
+        g.nonce = reqs[i].nonce;
+        g.which = reqs[i].which;
+        g.ncalls = reqs[i].ncalls;
+        g.nchunks = (int)gauss_window(reqs[i].ncalls, level);
+        g.chunk_base = chunks;
+        g.out0 = reqs[i].out0;
+        g.out1 = reqs[i].out1;
+        g.out_stride = reqs[i].out_stride;
+        tile_prefix[i] = tiles;
+        chunk_prefix[i] = chunks;
+        tiles += (g.nchunks + kTile - 1) / kTile;
+        chunks += g.nchunks;
+    }
+    DevBuf<GaussStream> d_streams;
+    DevBuf<long long> d_tp, d_cp;
+    DevBuf<uint8_t> d_cnt;
+    DevBuf<uint64_t> d_vals;
+    DevBuf<int> d_pos, d_status;
+    if (d_streams.alloc(n) || d_tp.alloc(n) || d_cp.alloc(n) || d_cnt.alloc(chunks) || d_vals.alloc(chunks * 8) ||
+        d_pos.alloc(chunks) || d_status.alloc(n + 1))
+        return -1;
+    HIP_OK(hipMemcpyAsync(d_streams.get(), hs.data(), sizeof(GaussStream) * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d_tp.get(), tile_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(d_cp.get(), chunk_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
+    if (launch_fill_i32(d_pos.get(), -1, chunks, st) || launch_fill_i32(d_status.get(), 0, n + 1, st)) return -1;
+    if (launch_gauss_candidates(d_streams.get(), n, d_tp.get(), tiles, d_cnt.get(), d_vals.get(), d_status.get() + n,
+                                st))
+        return -1;
+    if (launch_gauss_walk(d_streams.get(), n, d_cnt.get(), d_pos.get(), d_status.get(), st)) return -1;
+    if (launch_gauss_scatter(d_streams.get(), n, d_cp.get(), chunks, d_cnt.get(), d_vals.get(), d_pos.get(), st))
+        return -1;
+    std::vector<int> status(n + 1);
+    HIP_OK(hipMemcpyAsync(status.data(), d_status.get(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (status[n] & 2) {
+        ipfe_set_error("Gaussian sampler: a chunk needed more than 46 uniform words (reference reads past its buffer)");
+        return -1;
+    }
+    *overflow = 0;
+    for (int i = 0; i < n; ++i) *overflow |= status[i];
+    return 0;
+}
+
+int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch_budget)
+{
+    const size_t per_chunk = 1 + 64 + 4;
+    size_t i = 0;
+    while (i < reqs.size()) {
+        // group streams so that the chunk scratch stays within budget
+        size_t j = i, bytes = 0;
+        while (j < reqs.size()) {
+            size_t b = (size_t)gauss_window(reqs[j].ncalls, 0) * per_chunk;
+            if (j > i && bytes + b > scratch_budget) break;
+            bytes += b;
+            ++j;
+        }
+        int level = 0;
+        for (;;) {
+            int overflow = 0;
+            if (run_gauss_group(reqs.data() + i, (int)(j - i), level, st, &overflow)) return -1;
+            if (!overflow) break;
+            if (++level > 6) {
+                ipfe_set_error("Gaussian sampler: speculative window exhausted");
+                return -1;
+            }
+        }
+        i = j;
+    }
+    return 0;
+}
+
+}  // namespace ipfe

@@ -1,0 +1,80 @@
+// runtime.h -- device state shared by the C ABI: uploaded constant tables,
+// device buffers with RAII, and the Gaussian-sampler driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "host_tables.h"
+#include "kernels.h"
+
+namespace ipfe {
+
+extern const uint32_t *g_aes_te;
+
+// Per-process device state (one GPU): constant tables resident in HBM.
+struct Device {
+    uint32_t *tw_all = nullptr;  // [4][3][N]
+    uint32_t *aes_te = nullptr;  // [1024]
+    NinvConsts ninv;
+    ScaleConsts scale;
+    BarrettConsts barrett;
+    hipStream_t stream = nullptr;
+};
+// Initialises on first use; returns nullptr (error set) on failure.
+Device *device();
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count)
+    {
+        reset();
+        if (count == 0) return 0;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            ipfe_set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+            return -1;
+        }
+        n = count;
+        return 0;
+    }
+    T *get() const { return p; }
+};
+
+// One Gaussian stream request for the sampler driver.
+struct StreamReq {
+    uint32_t rk[60];
+    uint16_t nonce;
+    int which;
+    int ncalls;
+    uint32_t *out0;
+    uint32_t *out1;
+    long long out_stride;
+};
+StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncalls, uint32_t *out0, uint32_t *out1,
+                      long long out_stride);
+
+// Samples every request (device outputs), bit-exact with the reference
+// sequential sampler; splits into groups that fit scratch_budget bytes.
+int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch_budget = (size_t)8 << 30);
+
+// Speculative window (chunks) for a stream of ncalls calls at margin level m.
+long long gauss_window(int ncalls, int m);
+
+int sync_stream(hipStream_t st);
+
+}  // namespace ipfe

@@ -1,0 +1,141 @@
+/*
+ * cufe_ipfe.h -- C ABI of libcufe_ipfe.so, an MI355X (gfx950) implementation of
+ * the RLWE inner-product functional-encryption path ("rlwe_sife") and its
+ * linear-SVM caller.  Plain C types only; every buffer is caller-owned host
+ * memory unless a function says "device".
+ *
+ * Parameters are the reference's only set (src/params.h:14-28): N = 4096,
+ * three RNS primes, L = 785, p = 50241.  Residue arrays are uint32 in
+ * [0, q_j], limb-major: poly[slot][limb][coefficient].
+ *
+ * Randomness: the reference seeds its AES-256-CTR streams from getrandom and
+ * leaves two streams uninitialised (src/rlwe_sife.c:40,120).  This library fixes
+ * the convention that makes it bit-exact with the reference CPU path:
+ *   setup:   s_i from AES-256-CTR(seed2, nonce 0), e_i from AES-256-CTR(seed3, nonce 0);
+ *   encrypt: message b of a batch uses nonce b: r and f_0 from (seed2, b),
+ *            f_0 .. f_{L-1} from (seed3, b).  Hence repeat <= 65536 per seed pair.
+ */
+#ifndef CUFE_IPFE_H
+#define CUFE_IPFE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUFE_SIFE_N 4096
+#define CUFE_SIFE_NMODULI 3
+#define CUFE_SIFE_L 785
+#define CUFE_SIFE_P 50241
+
+/* ===================================================================== */
+/* 1. Reference drop-in entry points                                      */
+/*    Same symbols, parameter layouts and call semantics as the reference */
+/*    GPU module in its PERF build (src/params.h:9 defines PERF, so the   */
+/*    trailing float* accumulates elapsed device milliseconds; NULL is    */
+/*    accepted).  Each call allocates, computes and frees its own device  */
+/*    memory, as the reference does.  They return void like the           */
+/*    reference; failures print to stderr and set cufe_last_error().      */
+/* ===================================================================== */
+
+/* Replaces rlwe_sife_setup_gpu (src/rlwe_sife_gpu.cu:19).  On entry mpk[785]
+ * holds the uniform polynomial a in the coefficient domain (what
+ * sample_polya(seed1) writes, src/rlwe_sife.c:97); on return mpk[785] = NTT(a),
+ * mpk[0..784] = a_hat . NTT(s_i) + NTT(e_i) and msk[i] = s_i (coefficients). */
+void rlwe_sife_setup_gpu(uint32_t mpk[CUFE_SIFE_L + 1][CUFE_SIFE_NMODULI][CUFE_SIFE_N],
+                         uint32_t msk[CUFE_SIFE_L][CUFE_SIFE_NMODULI][CUFE_SIFE_N], unsigned char *seed2,
+                         unsigned char *seed3, float *part2_time);
+
+/* Replaces rlwe_sife_encrypt_gpu (src/rlwe_sife_gpu.cu:102).  m is
+ * [repeat][785], c is [repeat][786][3][4096] in the coefficient domain. */
+void rlwe_sife_encrypt_gpu(uint32_t *m, uint32_t mpk[CUFE_SIFE_L + 1][CUFE_SIFE_NMODULI][CUFE_SIFE_N], uint32_t *c,
+                           unsigned char *seed2, unsigned char *seed3, int repeat, float *part2_time);
+
+/* Replaces rlwe_sife_keygen_gpu (src/rlwe_sife_gpu.cu:191).  y is
+ * [repeat][785], sk_y is [repeat][3][4096]. */
+void rlwe_sife_keygen_gpu(const uint32_t *y, const uint32_t msk[CUFE_SIFE_L][CUFE_SIFE_NMODULI][CUFE_SIFE_N],
+                          uint32_t *sk_y, int repeat, float *part2_time);
+
+/* Replaces rlwe_sife_decrypt_gmp_gpu (src/rlwe_sife_gpu.cu:231): one
+ * ciphertext against repeat keys; d_y [repeat][3][4096] is the RNS result that
+ * round_extract_gmp2 turns into the inner product. */
+void rlwe_sife_decrypt_gmp_gpu(uint32_t c[CUFE_SIFE_L + 1][CUFE_SIFE_NMODULI][CUFE_SIFE_N], const uint32_t *y,
+                               uint32_t *sk_y, uint32_t *d_y, int repeat, float *part2_time);
+
+/* ===================================================================== */
+/* 2. Caller-side helpers the reference runs on the CPU with GMP          */
+/* ===================================================================== */
+
+/* sample_polya (src/sample.c:56) on the GPU: a[3][4096]. Returns 0 or -1. */
+int cufe_sample_polya(const unsigned char seed[32], uint32_t *a);
+
+/* round_extract_gmp2 (src/rlwe_sife.c:334-393) without GMP: CRT of coefficient
+ * 0 of d_y[3][4096] and round(p x / Q), as a double. */
+double cufe_round_extract2(const uint32_t *d_y);
+
+/* Plain linear SVM model (the fields of libsvm's struct svm_model that
+ * svm_predict_gpu reads, svm/svm.h:52-72); support vectors dense [l][785]. */
+typedef struct {
+    int nr_class;
+    int l;                  /* total support vectors */
+    const int *nSV;         /* [nr_class] */
+    const int *label;       /* [nr_class] */
+    const double *sv_coef;  /* [nr_class-1][l] row-major */
+    const double *rho;      /* [nr_class*(nr_class-1)/2] */
+    const uint32_t *sv;     /* [l][785] dense, feature index = slot */
+} cufe_svm_model;
+
+/* svm_predict_gpu = svm_predict5 (main.c:270-390) for one ciphertext
+ * c[786][3][4096]: keygen for every SV, decryption, round_extract, vote.
+ * Returns the label; kvalue (optional, [l]) receives the decrypted inner
+ * products.  The plain per-call form: keys are recomputed on every call. */
+double cufe_svm_predict_gpu(const cufe_svm_model *model,
+                            uint32_t c[CUFE_SIFE_L + 1][CUFE_SIFE_NMODULI][CUFE_SIFE_N],
+                            const uint32_t msk[CUFE_SIFE_L][CUFE_SIFE_NMODULI][CUFE_SIFE_N], double *kvalue);
+
+/* ===================================================================== */
+/* 3. Status-returning API: 0 on success, -1 on failure with the reason   */
+/*    in cufe_last_error() (thread-local).                                */
+/* ===================================================================== */
+
+const char *cufe_last_error(void);
+/* Device name, CU count and HBM size of the current GPU. */
+int cufe_device_info(char *name, int name_len, int *cu_count, uint64_t *hbm_bytes);
+
+/* Single operations on host buffers (one call = one batched launch). */
+int cufe_ntt_forward(uint32_t *polys, int npoly);  /* [npoly][3][4096] in place: CT_forward */
+int cufe_ntt_inverse(uint32_t *polys, int npoly);  /* GS_reverse incl. N^-1 */
+/* nstreams independent AES-256-CTR Gaussian streams, stream t keyed by
+ * keys[32 t] with nonces[t]; ncalls consecutive 4096-sample calls each.
+ * which = 1 (sigma_1, k=266), 2 (k=304214978), 3 (k=608429953).
+ * out: [nstreams][ncalls][3][4096]. */
+int cufe_gauss_sample(const unsigned char *keys, const uint16_t *nonces, int nstreams, int which, int ncalls,
+                      uint32_t *out);
+int cufe_aes256ctr(const unsigned char key[32], uint16_t nonce, uint32_t ctr0, uint32_t nblocks, unsigned char *out);
+/* CRT + rounding of coefficient coef of nvec RNS polys d[nvec][3][4096]. */
+int cufe_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r);
+
+/* The scheme at a vector length l <= 785 (uneven-length batching).  With
+ * zero-padding, a length-l encryption equals the first l polynomials plus c_L
+ * of the length-785 one (the noise order does not depend on x), so these are
+ * the reference functions restricted to l slots.  Layouts: mpk [l+1][3][N]
+ * (index l = a_hat), msk [l][3][N], m [repeat][l], c [repeat][l+1][3][N]
+ * (index l = c_L), y [repeat][l], sk / d [repeat][3][N].  Encryption of
+ * message b uses nonce nonce0 + b. */
+int cufe_sife_setup(uint32_t *mpk, uint32_t *msk, const unsigned char seed2[32], const unsigned char seed3[32], int l);
+int cufe_sife_encrypt(const uint32_t *m, const uint32_t *mpk, uint32_t *c, const unsigned char seed2[32],
+                      const unsigned char seed3[32], int repeat, int l, int nonce0);
+int cufe_sife_keygen(const uint32_t *y, const uint32_t *msk, uint32_t *sk, int repeat, int l);
+int cufe_sife_decrypt(const uint32_t *c, const uint32_t *y, const uint32_t *sk, uint32_t *d, int repeat, int l);
+
+/* Device-resident timing of the single operations: runs `iters` launches of
+ * the kernel over npoly synthetic polys / nstreams streams already in HBM and
+ * returns the mean milliseconds per launch measured with HIP events. */
+int cufe_time_ntt(int npoly, int inverse, int iters, float *ms_per_launch);
+int cufe_time_gauss(int nstreams, int which, int iters, float *ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -155,6 +155,7 @@ int do_encrypt(const uint32_t *m, const uint32_t *mpk, uint32_t *c, const unsign
             msgs[b].rhat = d_r.get() + (size_t)b * kPoly;
             msgs[b].rhatp = d_rp.get() + (size_t)b * kPoly;
             msgs[b].c = cb;
+            msgs[b].cl = cb + (size_t)l * kPoly;
             msgs[b].len = l;
             for (int i = 0; i <= l; ++i) items.push_back(EncItem{b, i});
         }

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(NTT_THREADS) void k_enc_poly(const EncItem *__restr
     const uint32_t *pk = mpk + ((size_t)(is_cl ? mpk_l : i) * 3 + s) * IPFE_N;
     const uint32_t *rh = m.rhat + (size_t)s * IPFE_N;
     const uint32_t *rhp = m.rhatp + (size_t)s * IPFE_N;
-    uint32_t *c = m.c + ((size_t)i * 3 + s) * IPFE_N;
+    uint32_t *c = is_cl ? m.cl + (size_t)s * IPFE_N : m.c + ((size_t)i * 3 + s) * IPFE_N;
     uint32_t x[16], rv[16], rvp[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {

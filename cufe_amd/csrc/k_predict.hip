@@ -1,0 +1,327 @@
+// k_predict.hip -- batched IPFE-SVM prediction: every (row, support vector)
+// pair of svm_predict_cpu / svm_predict_gpu (main.c:137-390) at once.
+//
+// Only coefficient 0 of each decryption is used (main.c:205, src/rlwe_sife.c:380),
+// so for row r and key v, limb s:
+//   d0 = sub_mod( fold_i y_v[i] c_i[0],  (c_L * sk_v)[0] )
+// where the negacyclic coefficient 0 is a plain dot product
+//   (c_L * sk)[0] = c_0 sk_0 - sum_{j>=1} c_j sk_{N-j} = sum_j c_j s'_j,
+//   s'_0 = sk_0,  s'_j = -sk_{N-j}.
+// Over all rows and keys that is a rows x keys x 4096 modular contraction per
+// limb.  It runs on the fp64 FMA pipe: operands are mapped to the balanced
+// range (-q/2, q/2], every partial sum stays an exact integer below 2^53, and
+// the accumulators are reduced exactly (fma(-rint(acc/q), q, acc)) every
+// `kb` k-steps, where kb comes from the largest key coefficient.  Functional
+// keys of small vectors y built on the sigma_1 master key are small, which makes
+// kb large.  The sum part folds only the non-zero y_i with the reference's
+// sequential mod_prime rule; with the reference's length L = 785 it always ends
+// on zero slots and is canonical, so d0 does not depend on which
+// representative the reference's inverse NTT returns for the product.
+#include "crt_dev.h"
+#include "kernels_predict.h"
+
+namespace ipfe {
+
+// ---------------------------------------------------------------------------
+// Keys: sk [S][3][N] canonical -> s' as balanced doubles, layout [3][S][N]; and
+// the largest |s'| per limb (atomicMax over uint).
+// ---------------------------------------------------------------------------
+__global__ void k_contract_keys(const uint32_t *__restrict__ sk, int S, double *__restrict__ kd,
+                                unsigned int *__restrict__ maxabs)
+{
+    const int v = blockIdx.y, s = blockIdx.z;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t q = qmod(s);
+    const uint32_t *src = sk + ((size_t)v * 3 + s) * IPFE_N;
+    uint32_t t = (j == 0) ? src[0] : src[IPFE_N - j];
+    if (t >= q) t -= q;
+    // s'_j = +sk_0 or -sk_{N-j}, as a balanced residue
+    int64_t val = (j == 0) ? (int64_t)t : -(int64_t)t;
+    if (val > (int64_t)(q / 2)) val -= q;
+    if (val < -(int64_t)(q / 2)) val += q;
+    kd[((size_t)s * S + v) * IPFE_N + j] = (double)val;
+    unsigned int a = (unsigned int)(val < 0 ? -val : val);
+    // wave-level max then one atomic per wave
+    for (int d = 32; d >= 1; d >>= 1) a = max(a, (unsigned int)__shfl_xor((int)a, d, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxabs + s, a);
+}
+
+// rows c_L [R][3][N] canonical (stride row_stride words between rows) -> int32
+// balanced [3][R][N].
+__global__ void k_rows_balanced(const uint32_t *__restrict__ cl, size_t row_stride, int R, int32_t *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per_row = (size_t)3 * IPFE_N;
+    if (i >= (size_t)R * per_row) return;
+    const int r = (int)(i / per_row);
+    const int rem = (int)(i % per_row);
+    const int s = rem / IPFE_N, j = rem % IPFE_N;
+    const uint32_t q = qmod(s);
+    uint32_t t = cl[(size_t)r * row_stride + rem];
+    if (t >= q) t -= q;
+    int64_t v = (t > q / 2) ? (int64_t)t - q : (int64_t)t;
+    out[((size_t)s * R + r) * IPFE_N + j] = (int32_t)v;
+}
+
+// ---------------------------------------------------------------------------
+// Contraction: D[r][v] = sum_j A[r][j] B[v][j] mod q for one limb.
+// Block tile 128 rows x 128 keys, k-step 16, thread tile 8 x 8 (rows ty+16i,
+// keys tx+16j), fp64 accumulators reduced every kb k-steps.
+// ---------------------------------------------------------------------------
+constexpr int CT_BM = 128, CT_BN = 128, CT_BK = 16;
+
+__device__ __forceinline__ double reduce_bal(double acc, double qd, double qinv)
+{
+    return __fma_rn(-rint(acc * qinv), qd, acc);
+}
+
+__global__ __launch_bounds__(256) void k_contract(const int32_t *__restrict__ A, const double *__restrict__ B, int R,
+                                                   int S, uint32_t q, int kb, uint32_t *__restrict__ D)
+{
+    __shared__ double As[CT_BK][CT_BM + 1];
+    __shared__ double Bs[CT_BK][CT_BN + 1];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int n0 = blockIdx.x * CT_BN, m0 = blockIdx.y * CT_BM;
+    const double qd = (double)q, qinv = 1.0 / (double)q;
+
+    double acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0;
+
+    // loader mapping: 2 threads per row/key, 8 consecutive k each
+    const int lrow = tid >> 1, lk = (tid & 1) * 8;
+    const bool arow_ok = (m0 + lrow) < R;
+    const bool bkey_ok = (n0 + lrow) < S;
+    const int32_t *Ap = A + (size_t)(arow_ok ? m0 + lrow : 0) * IPFE_N + lk;
+    const double *Bp = B + (size_t)(bkey_ok ? n0 + lrow : 0) * IPFE_N + lk;
+
+    int steps = 0;
+    for (int k0 = 0; k0 < IPFE_N; k0 += CT_BK) {
+        {
+            int4 a0 = *reinterpret_cast<const int4 *>(Ap + k0);
+            int4 a1 = *reinterpret_cast<const int4 *>(Ap + k0 + 4);
+            const int av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+            for (int t = 0; t < 8; ++t) As[lk + t][lrow] = arow_ok ? (double)av[t] : 0.0;
+            const double2 *bp2 = reinterpret_cast<const double2 *>(Bp + k0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                double2 b = bp2[t];
+                Bs[lk + 2 * t][lrow] = bkey_ok ? b.x : 0.0;
+                Bs[lk + 2 * t + 1][lrow] = bkey_ok ? b.y : 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < CT_BK; ++kk) {
+            double a[8], b[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = __fma_rn(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+        if (++steps == kb) {
+            steps = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[i][j] = reduce_bal(acc[i][j], qd, qinv);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int r = m0 + ty + 16 * i;
+        if (r >= R) continue;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int v = n0 + tx + 16 * j;
+            if (v >= S) continue;
+            double x = reduce_bal(acc[i][j], qd, qinv);
+            long long t = (long long)x;
+            t %= (long long)q;
+            if (t < 0) t += q;
+            D[(size_t)r * S + v] = (uint32_t)t;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kvalue[r][v] = round(p * CRT(d0) / Q) with d0_s = sub_mod(fold_i y_v[i] c0[r][i][s], D_s[r][v]).
+// c0 [R][l][3]: coefficient 0 of c_0 .. c_{l-1} per row.  Sets *flag when a
+// sum part ends on q and the product residue is 0 (the one case where the
+// reference's representative of the product would matter).
+// ---------------------------------------------------------------------------
+__global__ void k_kvalue(const uint32_t *__restrict__ D, const uint32_t *__restrict__ c0, int l, SparseY y, int R,
+                         int S, uint32_t *__restrict__ kval, int *__restrict__ flag)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)R * S) return;
+    const int r = (int)(t / S), v = (int)(t % S);
+    const uint32_t *cr = c0 + (size_t)r * l * 3;
+    uint32_t d[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const uint32_t q = qmod(s);
+        uint32_t acc = 0;
+        int last = -1;
+        for (int e = y.off[v]; e < y.off[v + 1]; ++e) {
+            const int i = y.idx[e];
+            if (i > last + 1 && acc == q) acc = 0;
+            const uint64_t ys = mod_prime_exact(y.val[e], s);
+            acc = mod_prime_exact(ys * cr[(size_t)i * 3 + s] + acc, s);
+            last = i;
+        }
+        if (last < y.len - 1 && acc == q) acc = 0;
+        const uint32_t dd = D[((size_t)s * R + r) * S + v];
+        if (acc == q && dd == 0) atomicOr(flag, 1);
+        d[s] = sub_q(acc, dd, q);
+    }
+    kval[t] = crt_round(crt_garner(d[0], d[1], d[2]));
+}
+
+// ---------------------------------------------------------------------------
+// Decision per row (main.c:338-389): products rounded, then summed in the
+// reference's order, one thread per row.
+// ---------------------------------------------------------------------------
+__global__ void k_decide(const uint32_t *__restrict__ kval, int R, int S, SvmModelDev m, double *__restrict__ labels)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const uint32_t *kv = kval + (size_t)r * S;
+    int vote[SVM_MAX_CLASS];
+    for (int i = 0; i < m.nr_class; ++i) vote[i] = 0;
+    int p = 0;
+    for (int i = 0; i < m.nr_class; ++i)
+        for (int j = i + 1; j < m.nr_class; ++j) {
+            double sum = 0.0;
+            const double *coef1 = m.sv_coef + (size_t)(j - 1) * S;
+            const double *coef2 = m.sv_coef + (size_t)i * S;
+            for (int k = 0; k < m.nSV[i]; ++k) {
+                const int sv = m.start[i] + k;
+                sum = __dadd_rn(sum, __dmul_rn(coef1[sv], (double)kv[sv]));
+            }
+            for (int k = 0; k < m.nSV[j]; ++k) {
+                const int sv = m.start[j] + k;
+                sum = __dadd_rn(sum, __dmul_rn(coef2[sv], (double)kv[sv]));
+            }
+            sum = __dsub_rn(sum, m.rho[p]);
+            if (sum > 0) ++vote[i]; else ++vote[j];
+            ++p;
+        }
+    int best = 0;
+    for (int i = 1; i < m.nr_class; ++i)
+        if (vote[i] > vote[best]) best = i;
+    labels[r] = (double)m.label[best];
+}
+
+// ---------------------------------------------------------------------------
+// Compact encryption: coefficient 0 of c_i for i < l and every row,
+//   c_i[0] = add_mod( N^-1 sum_k pk_i[k] r_hat[k], f_i[0] ) + floor(Q/p) x_i  (mod q)
+// (coefficient 0 of GS_reverse is N^-1 times the sum of its inputs).  One block
+// per (row, limb); r_hat staged in LDS; one wave per i.
+// f0buf [R][l][3] holds f_i[0] on entry and c_i[0] on exit.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_enc_coef0(const uint32_t *__restrict__ rhat, const uint32_t *__restrict__ mpk,
+                                                    const uint32_t *__restrict__ x, int l, uint32_t *__restrict__ f0buf,
+                                                    NinvConsts nc, ScaleConsts sc, BarrettConsts bc)
+{
+    __shared__ uint32_t rs[IPFE_N];
+    const int r = blockIdx.x, s = blockIdx.y;
+    const uint32_t q = qmod(s);
+    const uint32_t *rh = rhat + ((size_t)r * 3 + s) * IPFE_N;
+    for (int k = threadIdx.x; k < IPFE_N; k += blockDim.x) rs[k] = rh[k];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = wave; i < l; i += blockDim.x / 64) {
+        const uint32_t *pk = mpk + ((size_t)i * 3 + s) * IPFE_N;
+        // partial sums of products below 2^62, reduced before they can overflow
+        uint64_t acc = 0;
+        for (int k = lane; k < IPFE_N; k += 64) {
+            uint64_t pr = (uint64_t)pk[k] * rs[k];
+            acc += reduce64(pr, q, bc.mu_hi[s], bc.mu_lo[s]);
+        }
+        // acc < 64 q < 2^37
+        for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+        if (lane == 0) {
+            uint32_t sum = (uint32_t)(acc % q);
+            uint32_t v = mul_shoup(sum, nc.ninv[s], nc.ninvp[s], q);
+            uint32_t *cell = f0buf + ((size_t)r * l + i) * 3 + s;
+            uint32_t t = add_q(v, *cell, q);
+            const uint32_t mj = mod_prime_exact(x[(size_t)r * l + i], s);
+            const uint32_t madd = mod_prime_exact((uint64_t)mj * sc.scale[s], s);
+            t += madd;
+            *cell = t >= q ? t - q : t;
+        }
+    }
+}
+
+// Gather f_i[0] for every (row, i) from full noise polys is avoided: the compact
+// sampler writes coefficient 0 only (see predict.cpp).
+
+// ---------------------------------------------------------------------------
+int launch_contract_keys(const uint32_t *sk, int S, double *kd, unsigned int *maxabs, hipStream_t st)
+{
+    if (S <= 0) return 0;
+    hipLaunchKernelGGL(k_contract_keys, dim3(IPFE_N / 256, S, 3), dim3(256), 0, st, sk, S, kd, maxabs);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, hipStream_t st)
+{
+    if (R <= 0) return 0;
+    const size_t n = (size_t)R * 3 * IPFE_N;
+    hipLaunchKernelGGL(k_rows_balanced, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cl, row_stride, R, out);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_contract(const int32_t *A, const double *B, int R, int S, int s, int kb, uint32_t *D, hipStream_t st)
+{
+    if (R <= 0 || S <= 0) return 0;
+    if (kb < 1) {
+        ipfe_set_error("contraction: reduction interval %d < 1", kb);
+        return -1;
+    }
+    dim3 grid((S + CT_BN - 1) / CT_BN, (R + CT_BM - 1) / CT_BM);
+    hipLaunchKernelGGL(k_contract, grid, dim3(256), 0, st, A, B, R, S, qmod(s), kb, D);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_kvalue(const uint32_t *D, const uint32_t *c0, int l, const SparseY &y, int R, int S, uint32_t *kval,
+                  int *flag, hipStream_t st)
+{
+    const size_t n = (size_t)R * S;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_kvalue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, D, c0, l, y, R, S, kval, flag);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_decide(const uint32_t *kval, int R, int S, const SvmModelDev &m, double *labels, hipStream_t st)
+{
+    if (R <= 0) return 0;
+    hipLaunchKernelGGL(k_decide, dim3((R + 127) / 128), dim3(128), 0, st, kval, R, S, m, labels);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_enc_coef0(const uint32_t *rhat, const uint32_t *mpk, const uint32_t *x, int R, int l, uint32_t *f0buf,
+                     const NinvConsts &nc, const ScaleConsts &sc, const BarrettConsts &bc, hipStream_t st)
+{
+    if (R <= 0 || l <= 0) return 0;
+    hipLaunchKernelGGL(k_enc_coef0, dim3(R, 3), dim3(256), 0, st, rhat, mpk, x, l, f0buf, nc, sc, bc);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace ipfe

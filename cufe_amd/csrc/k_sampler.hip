@@ -258,16 +258,25 @@ __global__ void k_gauss_scatter(const GaussStream *__restrict__ streams, int nst
     if (slot >= n || off + slot >= IPFE_N) return;
     const int si = find_stream(chunk_prefix, nstreams, gc);
     const GaussStream *S = streams + si;
-    uint32_t *out = (call == 0) ? S->out0 : S->out1 + (long long)(call - 1) * S->out_stride;
+    if (S->coef0_only && off + slot != 0) return;
     const uint64_t v = vals[gc * 8 + slot];
     const uint64_t z = v & 0x7fffffffffffffffull;
     const bool neg = (v >> 63) != 0;
     // output residues (src/gauss.c:773-779)
+    uint32_t *out;
+    long long lstride;
+    if (S->coef0_only) {
+        out = S->out0 + 3ll * call;
+        lstride = 1;
+    } else {
+        out = ((call == 0) ? S->out0 : S->out1 + (long long)(call - 1) * S->out_stride) + off + slot;
+        lstride = IPFE_N;
+    }
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         uint32_t m = mod_prime_exact(z, s);
         if (neg) m = mod_prime_exact((uint32_t)(qmod(s) - m), s);
-        out[s * IPFE_N + off + slot] = m;
+        out[s * lstride] = m;
     }
 }
 

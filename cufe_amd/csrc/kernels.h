@@ -24,7 +24,8 @@ struct EncMsg {
     const uint32_t *x;      // message [len]
     const uint32_t *rhat;   // NTT(r) [3][N]
     const uint32_t *rhatp;  // Shoup companions of rhat
-    uint32_t *c;            // ciphertext [len+1][3][N]; holds the noise f on entry
+    uint32_t *c;            // c_0 .. c_{len-1} [len][3][N]; hold the noise f on entry
+    uint32_t *cl;           // c_L [3][N]; holds f_0 on entry
     int len;
     int pad;
 };
@@ -57,6 +58,8 @@ struct GaussStream {
     uint32_t *out0;          // [3][N] destination of call 0
     uint32_t *out1;          // destination of call 1
     long long out_stride;    // words between consecutive calls from call 1 on
+    int coef0_only;          // 1: write only sample 0 of each call, 3 words at out0 + 3*call
+    int pad;
 };
 
 // Candidate pass: for every chunk of every stream, AES + CDT + Bernoulli;

@@ -1,0 +1,28 @@
+// kernels_predict.h -- launchers of the batched prediction kernels (k_predict.hip).
+#pragma once
+#include "kernels.h"
+
+namespace ipfe {
+
+constexpr int SVM_MAX_CLASS = 16;
+
+// Device view of the plain SVM model (main.c:338-389 reads these fields).
+struct SvmModelDev {
+    int nr_class;
+    int nSV[SVM_MAX_CLASS];
+    int start[SVM_MAX_CLASS];
+    int label[SVM_MAX_CLASS];
+    const double *sv_coef;  // [nr_class-1][S]
+    const double *rho;      // [nr_class*(nr_class-1)/2]
+};
+
+int launch_contract_keys(const uint32_t *sk, int S, double *kd, unsigned int *maxabs, hipStream_t st);
+int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, hipStream_t st);
+int launch_contract(const int32_t *A, const double *B, int R, int S, int s, int kb, uint32_t *D, hipStream_t st);
+int launch_kvalue(const uint32_t *D, const uint32_t *c0, int l, const SparseY &y, int R, int S, uint32_t *kval,
+                  int *flag, hipStream_t st);
+int launch_decide(const uint32_t *kval, int R, int S, const SvmModelDev &m, double *labels, hipStream_t st);
+int launch_enc_coef0(const uint32_t *rhat, const uint32_t *mpk, const uint32_t *x, int R, int l, uint32_t *f0buf,
+                     const NinvConsts &nc, const ScaleConsts &sc, const BarrettConsts &bc, hipStream_t st);
+
+}  // namespace ipfe

@@ -75,7 +75,7 @@ int sync_stream(hipStream_t st)
 }
 
 StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncalls, uint32_t *out0, uint32_t *out1,
-                      long long out_stride)
+                      long long out_stride, int coef0_only)
 {
     StreamReq r;
     aes256_expand_key(key, r.rk);
@@ -85,6 +85,7 @@ StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncal
     r.out0 = out0;
     r.out1 = out1;
     r.out_stride = out_stride;
+    r.coef0_only = coef0_only;
     return r;
 }
 
@@ -116,6 +117,8 @@ static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t 
         g.out0 = reqs[i].out0;
         g.out1 = reqs[i].out1;
         g.out_stride = reqs[i].out_stride;
+        g.coef0_only = reqs[i].coef0_only;
+        g.pad = 0;
         tile_prefix[i] = tiles;
         chunk_prefix[i] = chunks;
         tiles += (g.nchunks + kTile - 1) / kTile;

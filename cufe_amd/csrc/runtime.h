@@ -64,9 +64,10 @@ struct StreamReq {
     uint32_t *out0;
     uint32_t *out1;
     long long out_stride;
+    int coef0_only;
 };
 StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncalls, uint32_t *out0, uint32_t *out1,
-                      long long out_stride);
+                      long long out_stride, int coef0_only = 0);
 
 // Samples every request (device outputs), bit-exact with the reference
 // sequential sampler; splits into groups that fit scratch_budget bytes.

@@ -201,3 +201,159 @@ def time_gauss(nstreams: int, which: int, iters: int) -> float:
     ms = C.c_float()
     _check(lib().cufe_time_gauss(nstreams, which, iters, C.byref(ms)), "time_gauss")
     return ms.value
+
+
+# ---- resident context and batched prediction -------------------------------------
+class SvmModel:
+    """Plain linear SVM model: the fields svm_predict_gpu reads (svm/svm.h:52-72),
+    support vectors dense [l][785] (feature index = slot)."""
+
+    def __init__(self, nr_class, nSV, label, sv_coef, rho, sv_dense):
+        self.nr_class = int(nr_class)
+        self.nSV = np.ascontiguousarray(nSV, dtype=np.int32)
+        self.label = np.ascontiguousarray(label, dtype=np.int32)
+        self.sv_coef = np.ascontiguousarray(sv_coef, dtype=np.float64).reshape(self.nr_class - 1, -1)
+        self.rho = np.ascontiguousarray(rho, dtype=np.float64)
+        sv = np.ascontiguousarray(sv_dense, dtype=np.uint32)
+        if sv.shape[1] < L:
+            sv = np.concatenate([sv, np.zeros((sv.shape[0], L - sv.shape[1]), np.uint32)], axis=1)
+        self.sv = np.ascontiguousarray(sv)
+        self.l = self.sv.shape[0]
+
+    def as_struct(self):
+        return _SvmModelC(self.nr_class, self.l, self.nSV.ctypes.data_as(C.POINTER(C.c_int)),
+                          self.label.ctypes.data_as(C.POINTER(C.c_int)),
+                          self.sv_coef.ctypes.data_as(C.POINTER(C.c_double)),
+                          self.rho.ctypes.data_as(C.POINTER(C.c_double)),
+                          self.sv.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+
+class _SvmModelC(C.Structure):
+    _fields_ = [("nr_class", C.c_int), ("l", C.c_int), ("nSV", C.POINTER(C.c_int)), ("label", C.POINTER(C.c_int)),
+                ("sv_coef", C.POINTER(C.c_double)), ("rho", C.POINTER(C.c_double)), ("sv", C.POINTER(C.c_uint32))]
+
+
+def _ctx_sigs(lb):
+    p = C.c_void_p
+    sig = {
+        "cufe_ctx_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+        "cufe_ctx_destroy": (None, [p]),
+        "cufe_ctx_setup": (C.c_int, [p, _u8p, _u8p, _u8p, C.c_int]),
+        "cufe_ctx_load_keys": (C.c_int, [p, _u32p, _u32p, C.c_int]),
+        "cufe_ctx_get_keys": (C.c_int, [p, _u32p, _u32p]),
+        "cufe_ctx_set_model": (C.c_int, [p, C.POINTER(_SvmModelC)]),
+        "cufe_ctx_rows_reserve": (C.c_int, [p, C.c_int, C.c_int]),
+        "cufe_ctx_encrypt_rows": (C.c_int, [p, _u32p, C.c_int, _u8p, _u8p, C.c_int, C.POINTER(C.c_float)]),
+        "cufe_ctx_rows_count": (C.c_int, [p]),
+        "cufe_ctx_predict": (C.c_int, [p, C.c_int, C.c_int, _f64p, C.c_void_p, C.POINTER(C.c_float)]),
+        "cufe_ctx_model_info": (C.c_int, [p, C.POINTER(C.c_int), C.POINTER(C.c_uint)]),
+        "cufe_ctx_get_model_keys": (C.c_int, [p, _u32p]),
+        "cufe_ctx_get_row": (C.c_int, [p, C.c_int, _u32p, _u32p]),
+        "cufe_svm_predict_gpu": (C.c_double, [C.POINTER(_SvmModelC), _u32p, _u32p, C.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lb, name)
+        f.restype = res
+        f.argtypes = args
+
+
+class Context:
+    """Resident keys, functional keys and encrypted rows on the GPU."""
+
+    def __init__(self):
+        lb = lib()
+        if not getattr(lb, "_ctx_sigs", False):
+            _ctx_sigs(lb)
+            lb._ctx_sigs = True
+        h = C.c_void_p()
+        _check(lb.cufe_ctx_create(C.byref(h)), "ctx_create")
+        self._h = h
+        self.l_key = 0
+        self.row_l = 0
+        self.model = None
+
+    def close(self):
+        if self._h:
+            lib().cufe_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setup(self, seedA, seed2, seed3, l: int = L):
+        _check(lib().cufe_ctx_setup(self._h, _seed(seedA), _seed(seed2), _seed(seed3), l), "ctx_setup")
+        self.l_key = l
+
+    def load_keys(self, mpk, msk):
+        mpk, msk = _u32(mpk), _u32(msk)
+        l = msk.shape[0]
+        _check(lib().cufe_ctx_load_keys(self._h, mpk, msk, l), "ctx_load_keys")
+        self.l_key = l
+
+    def get_keys(self):
+        mpk = np.empty((self.l_key + 1, NMODULI, N), np.uint32)
+        msk = np.empty((self.l_key, NMODULI, N), np.uint32)
+        _check(lib().cufe_ctx_get_keys(self._h, mpk, msk), "ctx_get_keys")
+        return mpk, msk
+
+    def set_model(self, model: SvmModel):
+        self._mstruct = model.as_struct()
+        _check(lib().cufe_ctx_set_model(self._h, C.byref(self._mstruct)), "ctx_set_model")
+        self.model = model
+
+    def rows_reserve(self, cap: int, l: int):
+        _check(lib().cufe_ctx_rows_reserve(self._h, cap, l), "ctx_rows_reserve")
+        self.row_l = l
+
+    def encrypt_rows(self, x, seed2, seed3, nonce0: int = 0) -> float:
+        x = _u32(np.atleast_2d(x))
+        assert x.shape[1] == self.row_l
+        ms = C.c_float()
+        _check(lib().cufe_ctx_encrypt_rows(self._h, x, x.shape[0], _seed(seed2), _seed(seed3), nonce0, C.byref(ms)),
+               "ctx_encrypt_rows")
+        return ms.value
+
+    def rows_count(self) -> int:
+        return lib().cufe_ctx_rows_count(self._h)
+
+    def predict(self, row0: int, nrows: int, want_kvalues: bool = False):
+        labels = np.empty(nrows, np.float64)
+        kv = np.empty((nrows, self.model.l), np.uint32) if want_kvalues else None
+        ms = (C.c_float * 4)()
+        _check(lib().cufe_ctx_predict(self._h, row0, nrows, labels, kv.ctypes.data if kv is not None else None, ms),
+               "ctx_predict")
+        return labels, kv, list(ms)
+
+    def model_info(self):
+        kb = (C.c_int * 3)()
+        mx = (C.c_uint * 3)()
+        _check(lib().cufe_ctx_model_info(self._h, kb, mx), "ctx_model_info")
+        return list(kb), list(mx)
+
+    def model_keys(self):
+        sk = np.empty((self.model.l, NMODULI, N), np.uint32)
+        _check(lib().cufe_ctx_get_model_keys(self._h, sk), "ctx_get_model_keys")
+        return sk
+
+    def get_row(self, row: int):
+        cl = np.empty((NMODULI, N), np.uint32)
+        c0 = np.empty((self.row_l, NMODULI), np.uint32)
+        _check(lib().cufe_ctx_get_row(self._h, row, cl, c0), "ctx_get_row")
+        return cl, c0
+
+
+def svm_predict_plain(model: SvmModel, c_full, msk) -> tuple:
+    """svm_predict_gpu (main.c:270-390) through the reference ABI on one full
+    ciphertext c [786][3][N] with msk [785][3][N]: (label, kvalues)."""
+    lb = lib()
+    if not getattr(lb, "_ctx_sigs", False):
+        _ctx_sigs(lb)
+        lb._ctx_sigs = True
+    ms = model.as_struct()
+    kv = np.empty(model.l, np.float64)
+    lab = lb.cufe_svm_predict_gpu(C.byref(ms), _u32(c_full), _u32(msk), kv.ctypes.data)
+    msg = lb.cufe_last_error()
+    return lab, kv

@@ -46,9 +46,10 @@ __global__ void k_contract_keys(const uint32_t *__restrict__ sk, int S, double *
     if ((threadIdx.x & 63) == 0) atomicMax(maxabs + s, a);
 }
 
-// rows c_L [R][3][N] canonical (stride row_stride words between rows) -> int32
-// balanced [3][R][N].
-__global__ void k_rows_balanced(const uint32_t *__restrict__ cl, size_t row_stride, int R, int32_t *__restrict__ out)
+// rows c_L [R][3][N] canonical (row_stride words between rows) -> int32 balanced,
+// written to out[(s * cap + r0 + r) * N + j] (the row store is [3][cap][N]).
+__global__ void k_rows_balanced(const uint32_t *__restrict__ cl, size_t row_stride, int R, int32_t *__restrict__ out,
+                                int cap, int r0)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t per_row = (size_t)3 * IPFE_N;
@@ -60,7 +61,7 @@ __global__ void k_rows_balanced(const uint32_t *__restrict__ cl, size_t row_stri
     uint32_t t = cl[(size_t)r * row_stride + rem];
     if (t >= q) t -= q;
     int64_t v = (t > q / 2) ? (int64_t)t - q : (int64_t)t;
-    out[((size_t)s * R + r) * IPFE_N + j] = (int32_t)v;
+    out[((size_t)s * cap + r0 + r) * IPFE_N + j] = (int32_t)v;
 }
 
 // ---------------------------------------------------------------------------
@@ -151,6 +152,29 @@ __global__ __launch_bounds__(256) void k_contract(const int32_t *__restrict__ A,
             D[(size_t)r * S + v] = (uint32_t)t;
         }
     }
+}
+
+// Exact integer contraction for key sets whose coefficients are too large for
+// exact fp64 partial sums: one thread per output, 64-bit products reduced one
+// by one.
+__global__ void k_contract_slow(const int32_t *__restrict__ A, const double *__restrict__ B, int R, int S, int s,
+                                BarrettConsts bc, uint32_t *__restrict__ D)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)R * S) return;
+    const int r = (int)(t / S), v = (int)(t % S);
+    const uint32_t q = qmod(s);
+    const int32_t *a = A + (size_t)r * IPFE_N;
+    const double *b = B + (size_t)v * IPFE_N;
+    uint64_t acc = 0;
+    for (int j = 0; j < IPFE_N; ++j) {
+        const uint32_t au = a[j] < 0 ? (uint32_t)(a[j] + (int64_t)q) : (uint32_t)a[j];
+        const long long bl = (long long)b[j];
+        const uint32_t bu = bl < 0 ? (uint32_t)(bl + (long long)q) : (uint32_t)bl;
+        acc += reduce64((uint64_t)au * bu, q, bc.mu_hi[s], bc.mu_lo[s]);
+        if (acc >= ((uint64_t)1 << 62)) acc %= q;
+    }
+    D[t] = (uint32_t)(acc % q);
 }
 
 // ---------------------------------------------------------------------------
@@ -275,11 +299,12 @@ int launch_contract_keys(const uint32_t *sk, int S, double *kd, unsigned int *ma
     return 0;
 }
 
-int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, hipStream_t st)
+int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, int cap, int r0, hipStream_t st)
 {
     if (R <= 0) return 0;
     const size_t n = (size_t)R * 3 * IPFE_N;
-    hipLaunchKernelGGL(k_rows_balanced, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cl, row_stride, R, out);
+    hipLaunchKernelGGL(k_rows_balanced, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cl, row_stride, R, out,
+                       cap, r0);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -293,6 +318,16 @@ int launch_contract(const int32_t *A, const double *B, int R, int S, int s, int 
     }
     dim3 grid((S + CT_BN - 1) / CT_BN, (R + CT_BM - 1) / CT_BM);
     hipLaunchKernelGGL(k_contract, grid, dim3(256), 0, st, A, B, R, S, qmod(s), kb, D);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_contract_slow(const int32_t *A, const double *B, int R, int S, int s, const BarrettConsts &bc, uint32_t *D,
+                         hipStream_t st)
+{
+    const size_t n = (size_t)R * S;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_contract_slow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, B, R, S, s, bc, D);
     HIP_OK(hipGetLastError());
     return 0;
 }

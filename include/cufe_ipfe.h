@@ -135,6 +135,47 @@ int cufe_sife_decrypt(const uint32_t *c, const uint32_t *y, const uint32_t *sk, 
 int cufe_time_ntt(int npoly, int inverse, int iters, float *ms_per_launch);
 int cufe_time_gauss(int nstreams, int which, int iters, float *ms_per_launch);
 
+/* ===================================================================== */
+/* 4. Resident-key context and batched prediction                          */
+/*    Keys, functional keys and encrypted rows stay in HBM; one predict     */
+/*    call classifies many rows against every support vector at once.       */
+/*    Outputs equal svm_predict_gpu's (main.c:270-390) row by row.          */
+/* ===================================================================== */
+
+typedef struct cufe_ctx cufe_ctx;
+
+int cufe_ctx_create(cufe_ctx **out);
+void cufe_ctx_destroy(cufe_ctx *ctx);
+/* Master keys of length l (<= 785) generated on the device: a =
+ * sample_polya(seedA), s_i from (seed2, nonce 0), e_i from (seed3, nonce 0). */
+int cufe_ctx_setup(cufe_ctx *ctx, const unsigned char seedA[32], const unsigned char seed2[32],
+                   const unsigned char seed3[32], int l);
+/* Master keys from host memory: mpk [l+1][3][4096], msk [l][3][4096]. */
+int cufe_ctx_load_keys(cufe_ctx *ctx, const uint32_t *mpk, const uint32_t *msk, int l);
+int cufe_ctx_get_keys(cufe_ctx *ctx, uint32_t *mpk, uint32_t *msk);
+/* Generates and keeps one functional key per support vector (rlwe_sife_keygen
+ * over the key length, src/rlwe_sife.c:215-245). */
+int cufe_ctx_set_model(cufe_ctx *ctx, const cufe_svm_model *model);
+/* Capacity of the encrypted-row store: cap rows of l slots each; l must exceed
+ * every feature index the model uses. */
+int cufe_ctx_rows_reserve(cufe_ctx *ctx, int cap, int l);
+/* Encrypts n rows x [n][l] and appends them to the store; row b uses nonce
+ * nonce0 + b (as rlwe_sife_encrypt_gpu numbers its batch).  ms: device time. */
+int cufe_ctx_encrypt_rows(cufe_ctx *ctx, const uint32_t *x, int n, const unsigned char seed2[32],
+                          const unsigned char seed3[32], int nonce0, float *ms);
+int cufe_ctx_rows_count(const cufe_ctx *ctx);
+/* Labels [nrows] of stored rows [row0, row0+nrows); kvalues (optional)
+ * [nrows][model l] = the decrypted inner products; ms (optional, 4 floats) =
+ * device milliseconds of contraction, key-value epilogue, vote, total. */
+int cufe_ctx_predict(cufe_ctx *ctx, int row0, int nrows, double *labels, uint32_t *kvalues, float *ms);
+/* Reduction interval of the fp64 contraction and the largest balanced
+ * functional-key coefficient, per limb. */
+int cufe_ctx_model_info(const cufe_ctx *ctx, int *kb, unsigned int *maxabs);
+/* Functional keys [model l][3][4096] and one stored row (c_L [3][4096] and
+ * coefficient 0 of c_0..c_{l-1} as [l][3]), for inspection. */
+int cufe_ctx_get_model_keys(const cufe_ctx *ctx, uint32_t *sk);
+int cufe_ctx_get_row(const cufe_ctx *ctx, int row, uint32_t *cl, uint32_t *c0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -130,7 +130,7 @@ int do_encrypt(const uint32_t *m, const uint32_t *mpk, uint32_t *c, const unsign
     HIP_OK(hipMemcpyAsync(d_mpk.get(), mpk, (l + 1) * kPoly * 4, hipMemcpyHostToDevice, st));
     const size_t ct_words = (size_t)(l + 1) * kPoly;
     // messages per group: bound ciphertext + sampler scratch to ~16 GiB
-    const size_t per_msg = ct_words * 4 + (size_t)gauss_window(l + 2, 0) * 69 + 2 * kPoly * 4;
+    const size_t per_msg = ct_words * 4 + gauss_bytes(3, l, 0) + gauss_bytes(2, 2, 0) + 2 * kPoly * 4;
     int group = (int)std::max<size_t>(1, ((size_t)16 << 30) / per_msg);
     for (int b0 = 0; b0 < repeat; b0 += group) {
         const int g = std::min(group, repeat - b0);

@@ -23,23 +23,35 @@
 namespace ipfe {
 
 // ---------------------------------------------------------------------------
-// Keys: sk [S][3][N] canonical -> s' as balanced doubles, layout [3][S][N]; and
-// the largest |s'| per limb (atomicMax over uint).
+// Keys: sk [S][3][N] canonical -> B [3][S][K] as balanced doubles (K >= N):
+//   columns j < N:   sign * s'_j  with s'_0 = sk_0, s'_j = -sk_{N-j};
+//   columns N + i:   y_v[i] mod q_s (the sum part of decryption), zero past the
+//                    SV support and when K == N.
+// sign = -1 gives D = sum_i y_i c_i[0] - (c_L * sk)[0] = d0 from one contraction
+// of the row [c_L, c_0[0], .., c_{l-1}[0]].  maxabs[s] = largest |B| per limb.
 // ---------------------------------------------------------------------------
-__global__ void k_contract_keys(const uint32_t *__restrict__ sk, int S, double *__restrict__ kd,
-                                unsigned int *__restrict__ maxabs)
+__global__ void k_contract_keys(const uint32_t *__restrict__ sk, int S, int K, int sign, SparseY y,
+                                double *__restrict__ kd, unsigned int *__restrict__ maxabs)
 {
     const int v = blockIdx.y, s = blockIdx.z;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t q = qmod(s);
-    const uint32_t *src = sk + ((size_t)v * 3 + s) * IPFE_N;
-    uint32_t t = (j == 0) ? src[0] : src[IPFE_N - j];
-    if (t >= q) t -= q;
-    // s'_j = +sk_0 or -sk_{N-j}, as a balanced residue
-    int64_t val = (j == 0) ? (int64_t)t : -(int64_t)t;
+    int64_t val = 0;
+    if (j < IPFE_N) {
+        const uint32_t *src = sk + ((size_t)v * 3 + s) * IPFE_N;
+        uint32_t t = (j == 0) ? src[0] : src[IPFE_N - j];
+        if (t >= q) t -= q;
+        val = (j == 0) ? (int64_t)t : -(int64_t)t;
+        val *= sign;
+    } else if (j < K) {
+        const int slot = j - IPFE_N;
+        for (int e = y.off[v]; e < y.off[v + 1]; ++e)
+            if (y.idx[e] == slot) val = mod_prime_exact(y.val[e], s);
+        if (val >= (int64_t)q) val -= q;
+    }
     if (val > (int64_t)(q / 2)) val -= q;
     if (val < -(int64_t)(q / 2)) val += q;
-    kd[((size_t)s * S + v) * IPFE_N + j] = (double)val;
+    if (j < K) kd[((size_t)s * S + v) * K + j] = (double)val;
     unsigned int a = (unsigned int)(val < 0 ? -val : val);
     // wave-level max then one atomic per wave
     for (int d = 32; d >= 1; d >>= 1) a = max(a, (unsigned int)__shfl_xor((int)a, d, 64));
@@ -47,9 +59,9 @@ __global__ void k_contract_keys(const uint32_t *__restrict__ sk, int S, double *
 }
 
 // rows c_L [R][3][N] canonical (row_stride words between rows) -> int32 balanced,
-// written to out[(s * cap + r0 + r) * N + j] (the row store is [3][cap][N]).
+// written to out[(s * cap + r0 + r) * ldr + j] (the row store is [3][cap][ldr]).
 __global__ void k_rows_balanced(const uint32_t *__restrict__ cl, size_t row_stride, int R, int32_t *__restrict__ out,
-                                int cap, int r0)
+                                int cap, int r0, int ldr)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t per_row = (size_t)3 * IPFE_N;
@@ -61,7 +73,28 @@ __global__ void k_rows_balanced(const uint32_t *__restrict__ cl, size_t row_stri
     uint32_t t = cl[(size_t)r * row_stride + rem];
     if (t >= q) t -= q;
     int64_t v = (t > q / 2) ? (int64_t)t - q : (int64_t)t;
-    out[((size_t)s * cap + r0 + r) * IPFE_N + j] = (int32_t)v;
+    out[((size_t)s * cap + r0 + r) * ldr + j] = (int32_t)v;
+}
+
+// Columns N .. ldr-1 of the row store: coefficient 0 of c_0 .. c_{l-1}
+// (c0 [R][l][3] canonical), balanced, zero-padded.
+__global__ void k_rows_c0(const uint32_t *__restrict__ c0, int R, int l, int32_t *__restrict__ out, int cap, int r0,
+                          int ldr)
+{
+    const int w = ldr - IPFE_N;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)R * 3 * w) return;
+    const int r = (int)(i / (3 * (size_t)w));
+    const int rem = (int)(i % (3 * (size_t)w));
+    const int s = rem / w, col = rem % w;
+    const uint32_t q = qmod(s);
+    int64_t v = 0;
+    if (col < l) {
+        uint32_t t = c0[((size_t)r * l + col) * 3 + s];
+        if (t >= q) t -= q;
+        v = (t > q / 2) ? (int64_t)t - q : (int64_t)t;
+    }
+    out[((size_t)s * cap + r0 + r) * ldr + IPFE_N + col] = (int32_t)v;
 }
 
 // ---------------------------------------------------------------------------
@@ -77,7 +110,8 @@ __device__ __forceinline__ double reduce_bal(double acc, double qd, double qinv)
 }
 
 __global__ __launch_bounds__(256) void k_contract(const int32_t *__restrict__ A, const double *__restrict__ B, int R,
-                                                   int S, uint32_t q, int kb, uint32_t *__restrict__ D)
+                                                   int S, int K, int lda, int ldb, uint32_t q, int kb,
+                                                   uint32_t *__restrict__ D)
 {
     __shared__ double As[CT_BK][CT_BM + 1];
     __shared__ double Bs[CT_BK][CT_BN + 1];
@@ -96,11 +130,11 @@ __global__ __launch_bounds__(256) void k_contract(const int32_t *__restrict__ A,
     const int lrow = tid >> 1, lk = (tid & 1) * 8;
     const bool arow_ok = (m0 + lrow) < R;
     const bool bkey_ok = (n0 + lrow) < S;
-    const int32_t *Ap = A + (size_t)(arow_ok ? m0 + lrow : 0) * IPFE_N + lk;
-    const double *Bp = B + (size_t)(bkey_ok ? n0 + lrow : 0) * IPFE_N + lk;
+    const int32_t *Ap = A + (size_t)(arow_ok ? m0 + lrow : 0) * lda + lk;
+    const double *Bp = B + (size_t)(bkey_ok ? n0 + lrow : 0) * ldb + lk;
 
     int steps = 0;
-    for (int k0 = 0; k0 < IPFE_N; k0 += CT_BK) {
+    for (int k0 = 0; k0 < K; k0 += CT_BK) {
         {
             int4 a0 = *reinterpret_cast<const int4 *>(Ap + k0);
             int4 a1 = *reinterpret_cast<const int4 *>(Ap + k0 + 4);
@@ -157,22 +191,21 @@ __global__ __launch_bounds__(256) void k_contract(const int32_t *__restrict__ A,
 // Exact integer contraction for key sets whose coefficients are too large for
 // exact fp64 partial sums: one thread per output, 64-bit products reduced one
 // by one.
-__global__ void k_contract_slow(const int32_t *__restrict__ A, const double *__restrict__ B, int R, int S, int s,
-                                BarrettConsts bc, uint32_t *__restrict__ D)
+__global__ void k_contract_slow(const int32_t *__restrict__ A, const double *__restrict__ B, int R, int S, int K,
+                                int lda, int ldb, int s, BarrettConsts bc, uint32_t *__restrict__ D)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)R * S) return;
     const int r = (int)(t / S), v = (int)(t % S);
     const uint32_t q = qmod(s);
-    const int32_t *a = A + (size_t)r * IPFE_N;
-    const double *b = B + (size_t)v * IPFE_N;
+    const int32_t *a = A + (size_t)r * lda;
+    const double *b = B + (size_t)v * ldb;
     uint64_t acc = 0;
-    for (int j = 0; j < IPFE_N; ++j) {
+    for (int j = 0; j < K; ++j) {
         const uint32_t au = a[j] < 0 ? (uint32_t)(a[j] + (int64_t)q) : (uint32_t)a[j];
         const long long bl = (long long)b[j];
         const uint32_t bu = bl < 0 ? (uint32_t)(bl + (long long)q) : (uint32_t)bl;
         acc += reduce64((uint64_t)au * bu, q, bc.mu_hi[s], bc.mu_lo[s]);
-        if (acc >= ((uint64_t)1 << 62)) acc %= q;
     }
     D[t] = (uint32_t)(acc % q);
 }
@@ -209,6 +242,15 @@ __global__ void k_kvalue(const uint32_t *__restrict__ D, const uint32_t *__restr
         d[s] = sub_q(acc, dd, q);
     }
     kval[t] = crt_round(crt_garner(d[0], d[1], d[2]));
+}
+
+// kvalue[r][v] = round(p * CRT(D_0, D_1, D_2) / Q) when the contraction already
+// produced d0.
+__global__ void k_crt_kvalue(const uint32_t *__restrict__ D, size_t n, uint32_t *__restrict__ kval)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    kval[t] = crt_round(crt_garner(D[t], D[n + t], D[2 * n + t]));
 }
 
 // ---------------------------------------------------------------------------
@@ -291,43 +333,56 @@ __global__ __launch_bounds__(256) void k_enc_coef0(const uint32_t *__restrict__ 
 // sampler writes coefficient 0 only (see predict.cpp).
 
 // ---------------------------------------------------------------------------
-int launch_contract_keys(const uint32_t *sk, int S, double *kd, unsigned int *maxabs, hipStream_t st)
+int launch_contract_keys(const uint32_t *sk, int S, int K, int sign, const SparseY &y, double *kd, unsigned int *maxabs,
+                         hipStream_t st)
 {
     if (S <= 0) return 0;
-    hipLaunchKernelGGL(k_contract_keys, dim3(IPFE_N / 256, S, 3), dim3(256), 0, st, sk, S, kd, maxabs);
+    hipLaunchKernelGGL(k_contract_keys, dim3((K + 255) / 256, S, 3), dim3(256), 0, st, sk, S, K, sign, y, kd, maxabs);
     HIP_OK(hipGetLastError());
     return 0;
 }
 
-int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, int cap, int r0, hipStream_t st)
+int launch_rows_c0(const uint32_t *c0, int R, int l, int32_t *out, int cap, int r0, int ldr, hipStream_t st)
+{
+    const size_t n = (size_t)R * 3 * (ldr - IPFE_N);
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_rows_c0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c0, R, l, out, cap, r0, ldr);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, int cap, int r0, int ldr,
+                         hipStream_t st)
 {
     if (R <= 0) return 0;
     const size_t n = (size_t)R * 3 * IPFE_N;
     hipLaunchKernelGGL(k_rows_balanced, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cl, row_stride, R, out,
-                       cap, r0);
+                       cap, r0, ldr);
     HIP_OK(hipGetLastError());
     return 0;
 }
 
-int launch_contract(const int32_t *A, const double *B, int R, int S, int s, int kb, uint32_t *D, hipStream_t st)
+int launch_contract(const int32_t *A, const double *B, int R, int S, int K, int lda, int ldb, int s, int kb, uint32_t *D,
+                    hipStream_t st)
 {
     if (R <= 0 || S <= 0) return 0;
-    if (kb < 1) {
-        ipfe_set_error("contraction: reduction interval %d < 1", kb);
+    if (kb < 1 || K % CT_BK || lda % 4 || ldb % 2 || K > lda || K > ldb) {
+        ipfe_set_error("contraction: bad shape K=%d lda=%d ldb=%d kb=%d", K, lda, ldb, kb);
         return -1;
     }
     dim3 grid((S + CT_BN - 1) / CT_BN, (R + CT_BM - 1) / CT_BM);
-    hipLaunchKernelGGL(k_contract, grid, dim3(256), 0, st, A, B, R, S, qmod(s), kb, D);
+    hipLaunchKernelGGL(k_contract, grid, dim3(256), 0, st, A, B, R, S, K, lda, ldb, qmod(s), kb, D);
     HIP_OK(hipGetLastError());
     return 0;
 }
 
-int launch_contract_slow(const int32_t *A, const double *B, int R, int S, int s, const BarrettConsts &bc, uint32_t *D,
-                         hipStream_t st)
+int launch_contract_slow(const int32_t *A, const double *B, int R, int S, int K, int lda, int ldb, int s,
+                         const BarrettConsts &bc, uint32_t *D, hipStream_t st)
 {
     const size_t n = (size_t)R * S;
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_contract_slow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, B, R, S, s, bc, D);
+    hipLaunchKernelGGL(k_contract_slow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, B, R, S, K, lda, ldb, s,
+                       bc, D);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -338,6 +393,14 @@ int launch_kvalue(const uint32_t *D, const uint32_t *c0, int l, const SparseY &y
     const size_t n = (size_t)R * S;
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_kvalue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, D, c0, l, y, R, S, kval, flag);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_crt_kvalue(const uint32_t *D, size_t n, uint32_t *kval, hipStream_t st)
+{
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_crt_kvalue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, D, n, kval);
     HIP_OK(hipGetLastError());
     return 0;
 }

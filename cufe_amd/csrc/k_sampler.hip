@@ -88,6 +88,81 @@ __device__ __forceinline__ bool bernoulli_bit(uint64_t b, double k2inv, const ui
     return (bits >> 63) != 0 || res == 0x3FF0000000000000ull;
 }
 
+// One candidate of a chunk held in LDS (96 little-endian words).  Lanes 8t..8t+7
+// of a wave evaluate candidates 0..7 of one chunk together: the uniform words
+// are shared through ballots, so every lane of the wave must call this.
+struct Cand {
+    uint64_t z;
+    uint32_t sgn;
+    bool acc;
+};
+
+__device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int which, bool valid, int *status)
+{
+    const uint64_t K = c_K[which], BF = c_BF[which];
+    const uint32_t UQ = (uint32_t)c_UQ[which];
+    const double K2INV = c_K2INV[which];
+    const int sh = (threadIdx.x & 63) & 56;
+    // uniform_sampler (src/gauss.c:609-703): the c-th word below UQ from byte 200 on
+    const uint32_t w0 = ch[50 + c], w1 = ch[58 + c], w2 = ch[66 + c];
+    const uint64_t m0 = __ballot(w0 < UQ), m1 = __ballot(w1 < UQ), m2 = __ballot(w2 < UQ);
+    const uint32_t mask = (uint32_t)((m0 >> sh) & 0xffu) | ((uint32_t)((m1 >> sh) & 0xffu) << 8) |
+                          ((uint32_t)((m2 >> sh) & 0xffu) << 16);
+    int pos = c;
+    if ((mask & 0xffu) != 0xffu) {
+        int seen = 0;
+        pos = -1;
+        for (int u = 0; u < 46; ++u) {
+            const bool kept = (u < 24) ? ((mask >> u) & 1u) : (ch[50 + u] < UQ);
+            if (kept) {
+                if (seen == c) { pos = u; break; }
+                ++seen;
+            }
+        }
+        if (pos < 0) {
+            if (valid) atomicOr(status, 2);
+            pos = 0;
+        }
+    }
+    const uint64_t w = ch[50 + pos];
+    uint64_t y = w - ((((w * BF) >> 32) + 1) * K);
+    y = y + (y >> 63) * K;
+
+    // cdt_sampler (src/gauss.c:122-154)
+    const int g = c >> 2, ln = c & 3;
+    const uint32_t *gb = ch + 25 * g;
+    const uint64_t r1 = (((uint64_t)gb[2 * ln + 1] << 32) | gb[2 * ln]) & 0x7fffffffffffffffull;
+    const uint64_t r2 = (((uint64_t)gb[8 + 2 * ln + 1] << 32) | gb[8 + 2 * ln]) & 0x7fffffffffffffffull;
+    uint32_t xs = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) xs += (r2 < c_CDT[i][1]) | ((r2 == c_CDT[i][1]) & (r1 < c_CDT[i][0]));
+
+    const uint64_t xk = (uint64_t)xs * (uint64_t)(uint32_t)K;
+    Cand out;
+    out.z = xk + y;
+    const uint64_t bin = (uint64_t)(uint32_t)(out.z + xk) * (uint64_t)(uint32_t)y;
+    const bool bit = bernoulli_bit(bin, K2INV, gb + 16, ln);
+    out.sgn = (ch[70] >> c) & 1u;
+    out.acc = bit && (out.z != 0 || out.sgn != 0);
+    return out;
+}
+
+// AES of nch chunks of one stream into LDS: chunk k at ks + 96 k uses
+// counters 24 (chunk_of(k)) .. +23.
+template <typename ChunkOf>
+__device__ __forceinline__ void aes_chunks(const GaussStream *S, const uint32_t *te, int nch, ChunkOf chunk_of,
+                                           uint32_t *ks)
+{
+    for (int g = threadIdx.x; g < nch * 24; g += CAND_THREADS) {
+        const int k = g / 24, blk = g - 24 * k;
+        const uint32_t ctr = (uint32_t)(chunk_of(k) * 24 + blk);
+        uint32_t o[4];
+        aes256_ctr_block(S->rk, te, S->nonce, ctr, o);
+        uint32_t *d = ks + k * CHUNK_WORDS + blk * 4;
+        d[0] = o[0]; d[1] = o[1]; d[2] = o[2]; d[3] = o[3];
+    }
+}
+
 __global__ __launch_bounds__(CAND_THREADS) void k_gauss_candidates(const GaussStream *__restrict__ streams, int nstreams,
                                                                     const long long *__restrict__ tile_prefix,
                                                                     const uint32_t *__restrict__ te_global,
@@ -101,90 +176,104 @@ __global__ __launch_bounds__(CAND_THREADS) void k_gauss_candidates(const GaussSt
     const long long chunk0 = ((long long)blockIdx.x - tile_prefix[si]) * TILE_CHUNKS;
     const int nch = (int)min((long long)TILE_CHUNKS, (long long)S->nchunks - chunk0);
     const int which = S->which;
-    const uint32_t nonce = S->nonce;
+    const bool store_vals = !S->coef0_only;
 
     aes_stage_tables(te, te_global);
     __syncthreads();
-    for (int g = threadIdx.x; g < nch * 24; g += CAND_THREADS) {
-        const int k = g / 24, blk = g - 24 * k;
-        const uint32_t ctr = (uint32_t)((chunk0 + k) * 24 + blk);
-        uint32_t o[4];
-        aes256_ctr_block(S->rk, te, nonce, ctr, o);
-        uint32_t *d = ks + k * CHUNK_WORDS + blk * 4;
-        d[0] = o[0]; d[1] = o[1]; d[2] = o[2]; d[3] = o[3];
-    }
+    aes_chunks(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks);
     __syncthreads();
 
-    const uint64_t K = c_K[which], BF = c_BF[which];
-    const uint32_t UQ = (uint32_t)c_UQ[which];
-    const double K2INV = c_K2INV[which];
-    const int lane = threadIdx.x & 63;
-    const int sh = lane & 56;
-
+    const int sh = (threadIdx.x & 63) & 56;
     for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += CAND_THREADS) {
         const int k = slot >> 3, c = slot & 7;
         const bool valid = k < nch;
-        const uint32_t *ch = ks + k * CHUNK_WORDS;
-        // uniform_sampler (src/gauss.c:609-703): the c-th word below UQ from byte 200 on
-        const uint32_t w0 = ch[50 + c], w1 = ch[58 + c], w2 = ch[66 + c];
-        const uint64_t m0 = __ballot(w0 < UQ), m1 = __ballot(w1 < UQ), m2 = __ballot(w2 < UQ);
-        const uint32_t mask = (uint32_t)((m0 >> sh) & 0xffu) | ((uint32_t)((m1 >> sh) & 0xffu) << 8) |
-                              ((uint32_t)((m2 >> sh) & 0xffu) << 16);
-        int pos = c;
-        if ((mask & 0xffu) != 0xffu) {
-            int seen = 0;
-            pos = -1;
-            for (int u = 0; u < 46; ++u) {
-                const bool kept = (u < 24) ? ((mask >> u) & 1u) : (ch[50 + u] < UQ);
-                if (kept) {
-                    if (seen == c) { pos = u; break; }
-                    ++seen;
-                }
-            }
-            if (pos < 0) {
-                if (valid) atomicOr(status, 2);
-                pos = 0;
-            }
-        }
-        const uint64_t w = ch[50 + pos];
-        uint64_t y = w - ((((w * BF) >> 32) + 1) * K);
-        y = y + (y >> 63) * K;
-
-        // cdt_sampler (src/gauss.c:122-154)
-        const int g = c >> 2, ln = c & 3;
-        const uint32_t *gb = ch + 25 * g;
-        const uint64_t r1 = (((uint64_t)gb[2 * ln + 1] << 32) | gb[2 * ln]) & 0x7fffffffffffffffull;
-        const uint64_t r2 = (((uint64_t)gb[8 + 2 * ln + 1] << 32) | gb[8 + 2 * ln]) & 0x7fffffffffffffffull;
-        uint32_t xs = 0;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) xs += (r2 < c_CDT[i][1]) | ((r2 == c_CDT[i][1]) & (r1 < c_CDT[i][0]));
-
-        const uint64_t xk = (uint64_t)xs * (uint64_t)(uint32_t)K;
-        const uint64_t z = xk + y;
-        const uint64_t bin = (uint64_t)(uint32_t)(z + xk) * (uint64_t)(uint32_t)y;
-        const bool bit = bernoulli_bit(bin, K2INV, gb + 16, ln);
-        const uint32_t sgn = (ch[70] >> c) & 1u;
-        const bool acc = bit && (z != 0 || sgn != 0);
-
-        const uint64_t am = __ballot(acc);
+        const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status);
+        const uint64_t am = __ballot(cd.acc);
         const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
         if (valid) {
             const long long gc = S->chunk_base + chunk0 + k;
             if (c == 0) cnt[gc] = (uint8_t)__popc(gm);
-            if (acc) vals[gc * 8 + __popc(gm & ((1u << c) - 1u))] = z | ((uint64_t)sgn << 63);
+            if (store_vals && cd.acc) vals[gc * 8 + __popc(gm & ((1u << c) - 1u))] = cd.z | ((uint64_t)cd.sgn << 63);
         }
     }
 }
 
-// One wave per stream.  Window of 64 lanes x 16 chunks.
+// Compact streams: sample 0 of every call.  Item t (a call of a compact
+// stream) has its first non-empty chunk in firsts[t]; the chunk is recomputed
+// and its first accepted candidate written as 3 residues at out0 + 3 call.
+constexpr int FIRST_TILE = 32;
+
+__global__ __launch_bounds__(CAND_THREADS) void k_gauss_firsts(const GaussStream *__restrict__ streams, int nstreams,
+                                                                const long long *__restrict__ first_prefix,
+                                                                const long long *__restrict__ firsts, long long nitems,
+                                                                const uint32_t *__restrict__ te_global,
+                                                                int *__restrict__ status)
+{
+    __shared__ uint32_t te[AES_TABLE_WORDS];
+    __shared__ uint32_t ks[FIRST_TILE * CHUNK_WORDS];
+    __shared__ int s_stream[FIRST_TILE];
+    __shared__ long long s_chunk[FIRST_TILE];
+    aes_stage_tables(te, te_global);
+    const long long t0 = (long long)blockIdx.x * FIRST_TILE;
+    if (threadIdx.x < FIRST_TILE) {
+        const long long t = t0 + threadIdx.x;
+        int si = -1;
+        long long ch = 0;
+        if (t < nitems) {
+            si = find_stream(first_prefix, nstreams, t);
+            ch = firsts[t] - streams[si].chunk_base;
+        }
+        s_stream[threadIdx.x] = si;
+        s_chunk[threadIdx.x] = ch;
+    }
+    __syncthreads();
+    // AES: 24 blocks per item, each item possibly from a different stream
+    for (int g = threadIdx.x; g < FIRST_TILE * 24; g += CAND_THREADS) {
+        const int k = g / 24, blk = g - 24 * k;
+        const int si = s_stream[k];
+        if (si < 0) continue;
+        const GaussStream *S = streams + si;
+        uint32_t o[4];
+        aes256_ctr_block(S->rk, te, S->nonce, (uint32_t)(s_chunk[k] * 24 + blk), o);
+        uint32_t *d = ks + k * CHUNK_WORDS + blk * 4;
+        d[0] = o[0]; d[1] = o[1]; d[2] = o[2]; d[3] = o[3];
+    }
+    __syncthreads();
+    const int sh = (threadIdx.x & 63) & 56;
+    const int k = threadIdx.x >> 3, c = threadIdx.x & 7;  // 32 items x 8 candidates = 256 threads
+    const int si = s_stream[k];
+    const bool valid = si >= 0;
+    const int which = valid ? streams[si].which : 1;
+    const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status);
+    const uint64_t am = __ballot(cd.acc && valid);
+    const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
+    if (valid && cd.acc && (gm & ((1u << c) - 1u)) == 0) {
+        const GaussStream *S = streams + si;
+        const long long call = t0 + k - first_prefix[si];
+        uint32_t *out = S->out0 + 3 * call;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            uint32_t m = mod_prime_exact(cd.z, s);
+            if (cd.sgn) m = mod_prime_exact((uint32_t)(qmod(s) - m), s);
+            out[s] = m;
+        }
+    }
+}
+
+// One wave per stream.  Window of 64 lanes x 16 chunks.  Full streams label
+// every used chunk with (call << 13 | first output position); compact streams
+// record only the first non-empty chunk of each call in firsts.
 __global__ __launch_bounds__(64) void k_gauss_walk(const GaussStream *__restrict__ streams, const uint8_t *__restrict__ cnt,
-                                                    int *__restrict__ pos, int *__restrict__ status)
+                                                    int *__restrict__ pos, long long *__restrict__ firsts,
+                                                    int *__restrict__ status)
 {
     const GaussStream *S = streams + blockIdx.x;
     const int lane = threadIdx.x;
     const long long base = S->chunk_base;
     const int nchunks = S->nchunks, ncalls = S->ncalls;
+    const bool compact = S->coef0_only != 0;
     int c = 0, call = 0, filled = 0;
+    bool first_done = false;
     while (call < ncalls) {
         if (c >= nchunks) {
             if (lane == 0) status[blockIdx.x] = 1;
@@ -207,6 +296,18 @@ __global__ __launch_bounds__(64) void k_gauss_walk(const GaussStream *__restrict
         }
         const uint32_t excl = incl - lsum;
         const uint32_t total = __shfl(incl, 63, 64);
+        if (compact && !first_done) {
+            const uint64_t nz = __ballot(lsum > 0);
+            if (nz) {
+                const int fl = __ffsll((unsigned long long)nz) - 1;
+                if (lane == fl) {
+                    int rr = 0;
+                    while (rr < 15 && v[rr] == 0) ++rr;
+                    firsts[S->first_base + call] = base + c + 16 * lane + rr;
+                }
+                first_done = true;
+            }
+        }
         const uint32_t need = 4096u - (uint32_t)filled;
         const uint64_t hit = __ballot(incl >= need);
         int e_chunk = -1;  // last chunk of this call, relative to c
@@ -222,18 +323,21 @@ __global__ __launch_bounds__(64) void k_gauss_walk(const GaussStream *__restrict
             }
             e_chunk = __shfl(e_chunk, hl, 64);
         }
-        const int last = (e_chunk >= 0) ? e_chunk : 1023;
-        uint32_t run = excl;
+        if (!compact) {
+            const int last = (e_chunk >= 0) ? e_chunk : 1023;
+            uint32_t run = excl;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int rel = 16 * lane + r;
-            const int ci = c + rel;
-            if (rel <= last && ci < nchunks) pos[base + ci] = (call << 13) | (int)((uint32_t)filled + run);
-            run += v[r];
+            for (int r = 0; r < 16; ++r) {
+                const int rel = 16 * lane + r;
+                const int ci = c + rel;
+                if (rel <= last && ci < nchunks) pos[base + ci] = (call << 13) | (int)((uint32_t)filled + run);
+                run += v[r];
+            }
         }
         if (e_chunk >= 0) {
             ++call;
             filled = 0;
+            first_done = false;
             c += e_chunk + 1;
         } else {
             filled += (int)total;
@@ -258,25 +362,16 @@ __global__ void k_gauss_scatter(const GaussStream *__restrict__ streams, int nst
     if (slot >= n || off + slot >= IPFE_N) return;
     const int si = find_stream(chunk_prefix, nstreams, gc);
     const GaussStream *S = streams + si;
-    if (S->coef0_only && off + slot != 0) return;
     const uint64_t v = vals[gc * 8 + slot];
     const uint64_t z = v & 0x7fffffffffffffffull;
     const bool neg = (v >> 63) != 0;
     // output residues (src/gauss.c:773-779)
-    uint32_t *out;
-    long long lstride;
-    if (S->coef0_only) {
-        out = S->out0 + 3ll * call;
-        lstride = 1;
-    } else {
-        out = ((call == 0) ? S->out0 : S->out1 + (long long)(call - 1) * S->out_stride) + off + slot;
-        lstride = IPFE_N;
-    }
+    uint32_t *out = ((call == 0) ? S->out0 : S->out1 + (long long)(call - 1) * S->out_stride) + off + slot;
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         uint32_t m = mod_prime_exact(z, s);
         if (neg) m = mod_prime_exact((uint32_t)(qmod(s) - m), s);
-        out[s * lstride] = m;
+        out[s * IPFE_N] = m;
     }
 }
 
@@ -331,11 +426,11 @@ int launch_gauss_candidates(const GaussStream *d_streams, int nstreams, const lo
     return 0;
 }
 
-int launch_gauss_walk(const GaussStream *d_streams, int nstreams, const uint8_t *cnt, int *pos, int *status,
-                      hipStream_t st)
+int launch_gauss_walk(const GaussStream *d_streams, int nstreams, const uint8_t *cnt, int *pos, long long *firsts,
+                      int *status, hipStream_t st)
 {
     if (nstreams <= 0) return 0;
-    hipLaunchKernelGGL(k_gauss_walk, dim3(nstreams), dim3(64), 0, st, d_streams, cnt, pos, status);
+    hipLaunchKernelGGL(k_gauss_walk, dim3(nstreams), dim3(64), 0, st, d_streams, cnt, pos, firsts, status);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -348,6 +443,16 @@ int launch_gauss_scatter(const GaussStream *d_streams, int nstreams, const long 
     const long long nthreads = nchunks_total * 8;
     hipLaunchKernelGGL(k_gauss_scatter, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, d_streams,
                        nstreams, d_chunk_prefix, nchunks_total, cnt, vals, pos);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_gauss_firsts(const GaussStream *d_streams, int nstreams, const long long *d_first_prefix,
+                        const long long *firsts, long long nitems, int *status, hipStream_t st)
+{
+    if (nitems <= 0) return 0;
+    hipLaunchKernelGGL(k_gauss_firsts, dim3((unsigned)((nitems + FIRST_TILE - 1) / FIRST_TILE)), dim3(CAND_THREADS), 0,
+                       st, d_streams, nstreams, d_first_prefix, firsts, nitems, g_aes_te, status);
     HIP_OK(hipGetLastError());
     return 0;
 }

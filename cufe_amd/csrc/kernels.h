@@ -60,6 +60,7 @@ struct GaussStream {
     long long out_stride;    // words between consecutive calls from call 1 on
     int coef0_only;          // 1: write only sample 0 of each call, 3 words at out0 + 3*call
     int pad;
+    long long first_base;    // compact streams: index of call 0 in the per-call first-chunk array
 };
 
 // Candidate pass: for every chunk of every stream, AES + CDT + Bernoulli;
@@ -68,8 +69,11 @@ int launch_gauss_candidates(const GaussStream *d_streams, int nstreams, const lo
                             long long ntiles, uint8_t *cnt, uint64_t *vals, int *status, hipStream_t st);
 // Walk: per stream, the call boundaries from the counts; writes per chunk
 // (call << 13 | offset) or -1.  status[stream] = 1 when the window was short.
-int launch_gauss_walk(const GaussStream *d_streams, int nstreams, const uint8_t *cnt, int *pos, int *status,
-                      hipStream_t st);
+int launch_gauss_walk(const GaussStream *d_streams, int nstreams, const uint8_t *cnt, int *pos, long long *firsts,
+                      int *status, hipStream_t st);
+// Compact streams: recompute the first non-empty chunk of every call and write sample 0.
+int launch_gauss_firsts(const GaussStream *d_streams, int nstreams, const long long *d_first_prefix,
+                        const long long *firsts, long long nitems, int *status, hipStream_t st);
 int launch_gauss_scatter(const GaussStream *d_streams, int nstreams, const long long *d_chunk_prefix,
                          long long nchunks_total, const uint8_t *cnt, const uint64_t *vals, const int *pos,
                          hipStream_t st);

@@ -16,11 +16,16 @@ struct SvmModelDev {
     const double *rho;      // [nr_class*(nr_class-1)/2]
 };
 
-int launch_contract_keys(const uint32_t *sk, int S, double *kd, unsigned int *maxabs, hipStream_t st);
-int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, int cap, int r0, hipStream_t st);
-int launch_contract_slow(const int32_t *A, const double *B, int R, int S, int s, const BarrettConsts &bc, uint32_t *D,
+int launch_contract_keys(const uint32_t *sk, int S, int K, int sign, const SparseY &y, double *kd, unsigned int *maxabs,
                          hipStream_t st);
-int launch_contract(const int32_t *A, const double *B, int R, int S, int s, int kb, uint32_t *D, hipStream_t st);
+int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, int cap, int r0, int ldr,
+                         hipStream_t st);
+int launch_rows_c0(const uint32_t *c0, int R, int l, int32_t *out, int cap, int r0, int ldr, hipStream_t st);
+int launch_contract_slow(const int32_t *A, const double *B, int R, int S, int K, int lda, int ldb, int s,
+                         const BarrettConsts &bc, uint32_t *D, hipStream_t st);
+int launch_contract(const int32_t *A, const double *B, int R, int S, int K, int lda, int ldb, int s, int kb, uint32_t *D,
+                    hipStream_t st);
+int launch_crt_kvalue(const uint32_t *D, size_t n, uint32_t *kval, hipStream_t st);
 int launch_kvalue(const uint32_t *D, const uint32_t *c0, int l, const SparseY &y, int R, int S, uint32_t *kval,
                   int *flag, hipStream_t st);
 int launch_decide(const uint32_t *kval, int R, int S, const SvmModelDev &m, double *labels, hipStream_t st);

@@ -34,14 +34,17 @@ struct cufe_ctx {
     DevBuf<uint32_t> y_val;
     SparseY y{};
     DevBuf<uint32_t> sk;          // [S][3][N]
-    DevBuf<double> kd;            // [3][S][N]
+    DevBuf<double> kd;            // [3][S][kkey]
+    int kkey = 0;                 // key columns: N (+ the y_i slots when fused)
+    bool fused = false;           // sum part folded into the contraction
     int kb[3] = {0, 0, 0};
     unsigned int maxabs[3] = {0, 0, 0};
     DevBuf<double> coef, rho;
     SvmModelDev mdev{};
     // rows
     int row_l = 0, cap = 0, nrows = 0;
-    DevBuf<int32_t> A;            // [3][cap][N]
+    int ldr = 0;                  // row-store columns: N + row_l rounded up to 16
+    DevBuf<int32_t> A;            // [3][cap][ldr]: c_L, then c_i[0] for i < row_l
     DevBuf<uint32_t> c0;          // [cap][row_l][3]
     // predict scratch
     int chunk_rows = 0;
@@ -187,8 +190,12 @@ extern "C" int cufe_ctx_set_model(cufe_ctx *c, const cufe_svm_model *m)
         ipfe_set_error("cufe_ctx_set_model: sum of nSV (%d) != l (%d)", nsv_sum, S);
         return -1;
     }
+    // With a zero slot after the last feature the reference's fold ends
+    // canonical, so the sum part joins the contraction (key = [-s', y]).
+    const bool fused = maxidx < L - 1;
+    const int kkey = fused ? IPFE_N + ((maxidx + 1 + 15) / 16) * 16 : IPFE_N;
     if (c->y_off.alloc(off.size()) || c->y_idx.alloc(idx.size()) || c->y_val.alloc(val.size()) ||
-        c->sk.alloc((size_t)S * kPoly) || c->kd.alloc((size_t)3 * S * IPFE_N))
+        c->sk.alloc((size_t)S * kPoly) || c->kd.alloc((size_t)3 * S * kkey))
         return -1;
     HIP_OK(hipMemcpyAsync(c->y_off.get(), off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(c->y_idx.get(), idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st));
@@ -201,7 +208,7 @@ extern "C" int cufe_ctx_set_model(cufe_ctx *c, const cufe_svm_model *m)
     DevBuf<unsigned int> d_max;
     if (d_max.alloc(3)) return -1;
     HIP_OK(hipMemsetAsync(d_max.get(), 0, 12, st));
-    if (launch_contract_keys(c->sk.get(), S, c->kd.get(), d_max.get(), st)) return -1;
+    if (launch_contract_keys(c->sk.get(), S, kkey, fused ? -1 : 1, c->y, c->kd.get(), d_max.get(), st)) return -1;
     HIP_OK(hipMemcpyAsync(c->maxabs, d_max.get(), 12, hipMemcpyDeviceToHost, st));
     // model coefficients
     const int npairs = m->nr_class * (m->nr_class - 1) / 2;
@@ -232,6 +239,8 @@ extern "C" int cufe_ctx_set_model(cufe_ctx *c, const cufe_svm_model *m)
     c->S = S;
     c->nr_class = m->nr_class;
     c->sv_maxidx = maxidx;
+    c->kkey = kkey;
+    c->fused = fused;
     return 0;
 }
 
@@ -242,9 +251,11 @@ extern "C" int cufe_ctx_rows_reserve(cufe_ctx *c, int cap, int l)
         ipfe_set_error("cufe_ctx_rows_reserve: cap %d / l %d invalid for key length %d", cap, l, c ? c->l_key : 0);
         return -1;
     }
-    if (c->A.alloc((size_t)3 * cap * IPFE_N) || c->c0.alloc((size_t)cap * l * 3)) return -1;
+    const int ldr = IPFE_N + ((l + 15) / 16) * 16;
+    if (c->A.alloc((size_t)3 * cap * ldr) || c->c0.alloc((size_t)cap * l * 3)) return -1;
     c->cap = cap;
     c->row_l = l;
+    c->ldr = ldr;
     c->nrows = 0;
     return 0;
 }
@@ -271,9 +282,10 @@ extern "C" int cufe_ctx_encrypt_rows(cufe_ctx *c, const uint32_t *x, int n, cons
     hipStream_t st = dev->stream;
     const int l = c->row_l;
     HIP_OK(hipEventRecord(c->ev[6], st));
-    // rows per group: sampler scratch dominates (about 53 KB per call)
-    const size_t per_row = (size_t)gauss_window(l + 2, 0) * 69 + 4 * kPoly * 4;
-    const int group = (int)std::max<size_t>(1, std::min<size_t>(8192, ((size_t)12 << 30) / per_row));
+    // rows per group: the S2 stream (r, f_0) keeps per-chunk values, the compact
+    // S3 stream only counts
+    const size_t per_row = gauss_bytes(2, 2, 0) + gauss_bytes(3, l, 1) + 4 * kPoly * 4 + (size_t)l * 16;
+    const int group = (int)std::max<size_t>(1, std::min<size_t>(16384, ((size_t)12 << 30) / per_row));
     DevBuf<uint32_t> d_x, d_r, d_rp, d_cl, d_f0;
     DevBuf<EncMsg> d_msgs;
     DevBuf<EncItem> d_items;
@@ -313,7 +325,8 @@ extern "C" int cufe_ctx_encrypt_rows(cufe_ctx *c, const uint32_t *x, int n, cons
         if (launch_enc_coef0(d_r.get(), c->mpk.get(), d_x.get(), g, l, d_f0.get(), dev->ninv, dev->scale, dev->barrett,
                              st))
             return -1;
-        if (launch_rows_balanced(d_cl.get(), kPoly, g, c->A.get(), c->cap, c->nrows + b0, st)) return -1;
+        if (launch_rows_balanced(d_cl.get(), kPoly, g, c->A.get(), c->cap, c->nrows + b0, c->ldr, st)) return -1;
+        if (launch_rows_c0(d_f0.get(), g, l, c->A.get(), c->cap, c->nrows + b0, c->ldr, st)) return -1;
         HIP_OK(hipMemcpyAsync(c->c0.get() + (size_t)(c->nrows + b0) * l * 3, d_f0.get(), (size_t)g * l * 3 * 4,
                               hipMemcpyDeviceToDevice, st));
         if (sync_stream(st)) return -1;
@@ -366,17 +379,21 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
         const int R = std::min(chunk, row0 + nrows - r0);
         HIP_OK(hipEventRecord(c->ev[0], st));
         for (int s = 0; s < 3; ++s) {
-            const int32_t *As = c->A.get() + ((size_t)s * c->cap + r0) * IPFE_N;
-            const double *Bs = c->kd.get() + (size_t)s * S * IPFE_N;
+            const int32_t *As = c->A.get() + ((size_t)s * c->cap + r0) * c->ldr;
+            const double *Bs = c->kd.get() + (size_t)s * S * c->kkey;
             uint32_t *Ds = c->D.get() + (size_t)s * R * S;
-            int rc = c->kb[s] >= 1 ? launch_contract(As, Bs, R, S, s, c->kb[s], Ds, st)
-                                   : launch_contract_slow(As, Bs, R, S, s, dev->barrett, Ds, st);
+            int rc = c->kb[s] >= 1 ? launch_contract(As, Bs, R, S, c->kkey, c->ldr, c->kkey, s, c->kb[s], Ds, st)
+                                   : launch_contract_slow(As, Bs, R, S, c->kkey, c->ldr, c->kkey, s, dev->barrett, Ds,
+                                                          st);
             if (rc) return -1;
         }
         HIP_OK(hipEventRecord(c->ev[1], st));
-        if (launch_kvalue(c->D.get(), c->c0.get() + (size_t)r0 * l * 3, l, c->y, R, S, c->kval.get(), c->flag.get(),
-                          st))
+        if (c->fused) {
+            if (launch_crt_kvalue(c->D.get(), (size_t)R * S, c->kval.get(), st)) return -1;
+        } else if (launch_kvalue(c->D.get(), c->c0.get() + (size_t)r0 * l * 3, l, c->y, R, S, c->kval.get(),
+                                 c->flag.get(), st)) {
             return -1;
+        }
         HIP_OK(hipEventRecord(c->ev[2], st));
         if (launch_decide(c->kval.get(), R, S, c->mdev, c->labels.get(), st)) return -1;
         HIP_OK(hipEventRecord(c->ev[3], st));
@@ -449,7 +466,7 @@ extern "C" int cufe_ctx_get_row(const cufe_ctx *c, int row, uint32_t *cl, uint32
     }
     std::vector<int32_t> a(kPoly);
     for (int s = 0; s < 3; ++s)
-        HIP_OK(hipMemcpy(a.data() + (size_t)s * IPFE_N, c->A.get() + ((size_t)s * c->cap + row) * IPFE_N,
+        HIP_OK(hipMemcpy(a.data() + (size_t)s * IPFE_N, c->A.get() + ((size_t)s * c->cap + row) * c->ldr,
                          IPFE_N * 4, hipMemcpyDeviceToHost));
     for (int s = 0; s < 3; ++s)
         for (int j = 0; j < IPFE_N; ++j) {

@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <math.h>
+
 #include <algorithm>
 #include <mutex>
 
@@ -89,59 +91,89 @@ StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncal
     return r;
 }
 
-// A call consumes 752.5 chunks on average (4096 samples at 8 x 0.6804 per
-// chunk) with a standard deviation of about 6.7 chunks; 770 per call plus 256
-// is > 40 sigma for any call count.  Level m doubles the margin m times.
-long long gauss_window(int ncalls, int m)
+// Chunks per call of the three samplers, measured on the reference algorithm:
+// S1 accepts 68.1% of its candidates (751.6 chunks per 4096 samples, standard
+// deviation 6.6); the Bernoulli step of S2 / S3 is distorted for large k (the
+// int-to-double trick of src/gauss.c:368-369 is wrong above 2^52), so S2 accepts
+// 53.8% (951.8 chunks, sd 10.1) and S3 38.7% (1323.4 chunks, sd 16.2).  The
+// window takes the mean plus 1.5% plus 8 standard deviations; level m doubles
+// the margin m times (a short window is detected and retried, never wrong).
+long long gauss_window(int which, int ncalls, int m)
 {
-    long long base = (long long)ncalls * 770 + 256;
-    long long extra = (long long)ncalls * 17 + 256;
-    for (int i = 0; i < m; ++i) extra *= 2;
-    return base - ((long long)ncalls * 17 + 256) + extra;
+    static const double mean[4] = {0.0, 751.6, 951.8, 1323.4};
+    static const double sd[4] = {0.0, 6.6, 10.1, 16.2};
+    const int w = (which >= 1 && which <= 3) ? which : 3;
+    const double n = (double)ncalls;
+    double margin = 0.015 * mean[w] * n + 8.0 * sd[w] * sqrt(n) + 64.0;
+    for (int i = 0; i < m; ++i) margin *= 2.0;
+    return (long long)ceil(mean[w] * n + margin);
+}
+
+size_t gauss_bytes(int which, int ncalls, int coef0_only)
+{
+    const long long ch = gauss_window(which, ncalls, 0);
+    return coef0_only ? (size_t)ch * 1 + (size_t)ncalls * 8 : (size_t)ch * (1 + 64 + 4);
 }
 
 static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t st, int *overflow)
 {
     constexpr int kTile = 64;
     std::vector<GaussStream> hs(n);
-    std::vector<long long> tile_prefix(n), chunk_prefix(n);
-    long long tiles = 0, chunks = 0;
-    for (int i = 0; i < n; ++i) {
+    std::vector<long long> tile_prefix(n), chunk_prefix(n), first_prefix(n);
+    long long tiles = 0, chunks = 0, full_chunks = 0, nfirst = 0;
+    // full streams first, so per-chunk values and labels cover only their chunks
+    std::vector<int> order;
+    for (int i = 0; i < n; ++i)
+        if (!reqs[i].coef0_only) order.push_back(i);
+    for (int i = 0; i < n; ++i)
+        if (reqs[i].coef0_only) order.push_back(i);
+    for (int k = 0; k < n; ++k) {
+        const int i = k;
+        const StreamReq &rq = reqs[order[k]];
         GaussStream &g = hs[i];
-        memcpy(g.rk, reqs[i].rk, sizeof g.rk);
-        g.nonce = reqs[i].nonce;
-        g.which = reqs[i].which;
-        g.ncalls = reqs[i].ncalls;
-        g.nchunks = (int)gauss_window(reqs[i].ncalls, level);
+        memcpy(g.rk, rq.rk, sizeof g.rk);
+        g.nonce = rq.nonce;
+        g.which = rq.which;
+        g.ncalls = rq.ncalls;
+        g.nchunks = (int)gauss_window(rq.which, rq.ncalls, level);
         g.chunk_base = chunks;
-        g.out0 = reqs[i].out0;
-        g.out1 = reqs[i].out1;
-        g.out_stride = reqs[i].out_stride;
-        g.coef0_only = reqs[i].coef0_only;
+        g.out0 = rq.out0;
+        g.out1 = rq.out1;
+        g.out_stride = rq.out_stride;
+        g.coef0_only = rq.coef0_only;
         g.pad = 0;
+        g.first_base = nfirst;
         tile_prefix[i] = tiles;
         chunk_prefix[i] = chunks;
+        first_prefix[i] = nfirst;
         tiles += (g.nchunks + kTile - 1) / kTile;
         chunks += g.nchunks;
+        if (g.coef0_only) nfirst += g.ncalls;
+        else full_chunks = chunks;
     }
+    // per-chunk values and labels are only kept up to the last full stream
     DevBuf<GaussStream> d_streams;
-    DevBuf<long long> d_tp, d_cp;
+    DevBuf<long long> d_tp, d_cp, d_fp, d_first;
     DevBuf<uint8_t> d_cnt;
     DevBuf<uint64_t> d_vals;
     DevBuf<int> d_pos, d_status;
-    if (d_streams.alloc(n) || d_tp.alloc(n) || d_cp.alloc(n) || d_cnt.alloc(chunks) || d_vals.alloc(chunks * 8) ||
-        d_pos.alloc(chunks) || d_status.alloc(n + 1))
+    if (d_streams.alloc(n) || d_tp.alloc(n) || d_cp.alloc(n) || d_fp.alloc(n) || d_cnt.alloc(chunks) ||
+        d_vals.alloc((size_t)full_chunks * 8 + 1) || d_pos.alloc((size_t)full_chunks + 1) ||
+        d_first.alloc((size_t)nfirst + 1) || d_status.alloc(n + 1))
         return -1;
     HIP_OK(hipMemcpyAsync(d_streams.get(), hs.data(), sizeof(GaussStream) * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(d_tp.get(), tile_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(d_cp.get(), chunk_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
-    if (launch_fill_i32(d_pos.get(), -1, chunks, st) || launch_fill_i32(d_status.get(), 0, n + 1, st)) return -1;
+    HIP_OK(hipMemcpyAsync(d_fp.get(), first_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
+    if (launch_fill_i32(d_pos.get(), -1, (size_t)full_chunks, st) || launch_fill_i32(d_status.get(), 0, n + 1, st))
+        return -1;
     if (launch_gauss_candidates(d_streams.get(), n, d_tp.get(), tiles, d_cnt.get(), d_vals.get(), d_status.get() + n,
                                 st))
         return -1;
-    if (launch_gauss_walk(d_streams.get(), n, d_cnt.get(), d_pos.get(), d_status.get(), st)) return -1;
-    if (launch_gauss_scatter(d_streams.get(), n, d_cp.get(), chunks, d_cnt.get(), d_vals.get(), d_pos.get(), st))
+    if (launch_gauss_walk(d_streams.get(), n, d_cnt.get(), d_pos.get(), d_first.get(), d_status.get(), st)) return -1;
+    if (launch_gauss_scatter(d_streams.get(), n, d_cp.get(), full_chunks, d_cnt.get(), d_vals.get(), d_pos.get(), st))
         return -1;
+    if (launch_gauss_firsts(d_streams.get(), n, d_fp.get(), d_first.get(), nfirst, d_status.get() + n, st)) return -1;
     std::vector<int> status(n + 1);
     HIP_OK(hipMemcpyAsync(status.data(), d_status.get(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -156,13 +188,12 @@ static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t 
 
 int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch_budget)
 {
-    const size_t per_chunk = 1 + 64 + 4;
     size_t i = 0;
     while (i < reqs.size()) {
         // group streams so that the chunk scratch stays within budget
         size_t j = i, bytes = 0;
         while (j < reqs.size()) {
-            size_t b = (size_t)gauss_window(reqs[j].ncalls, 0) * per_chunk;
+            size_t b = gauss_bytes(reqs[j].which, reqs[j].ncalls, reqs[j].coef0_only);
             if (j > i && bytes + b > scratch_budget) break;
             bytes += b;
             ++j;

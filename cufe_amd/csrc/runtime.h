@@ -73,8 +73,10 @@ StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncal
 // sequential sampler; splits into groups that fit scratch_budget bytes.
 int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch_budget = (size_t)8 << 30);
 
-// Speculative window (chunks) for a stream of ncalls calls at margin level m.
-long long gauss_window(int ncalls, int m);
+// Speculative window (chunks) for a stream of ncalls calls of sampler `which`
+// at margin level m, and the scratch bytes one stream needs at level 0.
+long long gauss_window(int which, int ncalls, int m);
+size_t gauss_bytes(int which, int ncalls, int coef0_only);
 
 int sync_stream(hipStream_t st);
 

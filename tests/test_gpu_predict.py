@@ -144,3 +144,28 @@ def test_predict_a1a_rows(gpu, po, keyed_ctx):
         for v in (0, 1, 290, 590):
             k, _ = oracle_pair(po, c_full, model.sv[v], msk, l)
             assert int(kv[r, v]) == k
+
+
+def test_predict_key_without_trailing_zero_slot(gpu, po):
+    """Key length = last feature + 1: the reference's fold can end on the
+    representative q, so the sum part runs as a separate exact fold."""
+    rng = np.random.default_rng(70)
+    l, R, S = 12, 3, 10
+    ctx = gpu.Context()
+    ctx.setup(h(b"uA"), h(b"uS"), h(b"uE"), l)
+    mpk, msk = ctx.get_keys()
+    model = make_model(gpu, rng, S, l)
+    model.sv[0, l - 1] = 5  # feature in the last key slot
+    model = gpu.SvmModel(model.nr_class, model.nSV, model.label, model.sv_coef, model.rho, model.sv)
+    x = rng.integers(0, 5, (R, l)).astype(np.uint32)
+    ctx.rows_reserve(R, l)
+    ctx.encrypt_rows(x, h(b"u2"), h(b"u3"), nonce0=0)
+    ctx.set_model(model)
+    labels, kv, _ = ctx.predict(0, R, want_kvalues=True)
+    for r in range(R):
+        c_full = po.encrypt(x[r], mpk, h(b"u2"), h(b"u3"), r)
+        for v in range(S):
+            sk = po.keygen(model.sv[v, :l], msk)
+            d = po.decrypt(c_full, model.sv[v, :l], sk)
+            assert int(kv[r, v]) == po.crt_round(int(d[0, 0]), int(d[1, 0]), int(d[2, 0]))
+    ctx.close()

@@ -15,6 +15,8 @@
 //      (call, first output position);
 //   3. scatter: every accepted candidate of a used chunk writes its three
 //      residues to its call's polynomial.
+#include <algorithm>
+
 #include "aes_dev.h"
 #include "kernels.h"
 
@@ -165,35 +167,37 @@ __device__ __forceinline__ void aes_chunks(const GaussStream *S, const uint32_t 
 
 __global__ __launch_bounds__(CAND_THREADS) void k_gauss_candidates(const GaussStream *__restrict__ streams, int nstreams,
                                                                     const long long *__restrict__ tile_prefix,
+                                                                    long long ntiles,
                                                                     const uint32_t *__restrict__ te_global,
                                                                     uint8_t *__restrict__ cnt, uint64_t *__restrict__ vals,
                                                                     int *__restrict__ status)
 {
     __shared__ uint32_t te[AES_TABLE_WORDS];
     __shared__ uint32_t ks[TILE_CHUNKS * CHUNK_WORDS];
-    const int si = __builtin_amdgcn_readfirstlane(find_stream(tile_prefix, nstreams, (long long)blockIdx.x));
-    const GaussStream *S = streams + si;
-    const long long chunk0 = ((long long)blockIdx.x - tile_prefix[si]) * TILE_CHUNKS;
-    const int nch = (int)min((long long)TILE_CHUNKS, (long long)S->nchunks - chunk0);
-    const int which = S->which;
-    const bool store_vals = !S->coef0_only;
-
     aes_stage_tables(te, te_global);
-    __syncthreads();
-    aes_chunks(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks);
-    __syncthreads();
-
     const int sh = (threadIdx.x & 63) & 56;
-    for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += CAND_THREADS) {
-        const int k = slot >> 3, c = slot & 7;
-        const bool valid = k < nch;
-        const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status);
-        const uint64_t am = __ballot(cd.acc);
-        const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
-        if (valid) {
-            const long long gc = S->chunk_base + chunk0 + k;
-            if (c == 0) cnt[gc] = (uint8_t)__popc(gm);
-            if (store_vals && cd.acc) vals[gc * 8 + __popc(gm & ((1u << c) - 1u))] = cd.z | ((uint64_t)cd.sgn << 63);
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int si = __builtin_amdgcn_readfirstlane(find_stream(tile_prefix, nstreams, tile));
+        const GaussStream *S = streams + si;
+        const long long chunk0 = (tile - tile_prefix[si]) * TILE_CHUNKS;
+        const int nch = (int)min((long long)TILE_CHUNKS, (long long)S->nchunks - chunk0);
+        const int which = S->which;
+        const bool store_vals = !S->coef0_only;
+        __syncthreads();  // tables staged / previous tile's reads of ks done
+        aes_chunks(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks);
+        __syncthreads();
+        for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += CAND_THREADS) {
+            const int k = slot >> 3, c = slot & 7;
+            const bool valid = k < nch;
+            const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status);
+            const uint64_t am = __ballot(cd.acc);
+            const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
+            if (valid) {
+                const long long gc = S->chunk_base + chunk0 + k;
+                if (c == 0) cnt[gc] = (uint8_t)__popc(gm);
+                if (store_vals && cd.acc)
+                    vals[gc * 8 + __popc(gm & ((1u << c) - 1u))] = cd.z | ((uint64_t)cd.sgn << 63);
+            }
         }
     }
 }
@@ -351,15 +355,15 @@ __global__ void k_gauss_scatter(const GaussStream *__restrict__ streams, int nst
                                 const uint8_t *__restrict__ cnt, const uint64_t *__restrict__ vals,
                                 const int *__restrict__ pos)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nchunks_total * 8;
+         t += (long long)gridDim.x * blockDim.x) {
     const long long gc = t >> 3;
     const int slot = (int)(t & 7);
-    if (gc >= nchunks_total) return;
     const int info = pos[gc];
-    if (info < 0) return;
+    if (info < 0) continue;
     const int n = cnt[gc];
     const int call = info >> 13, off = info & 0x1fff;
-    if (slot >= n || off + slot >= IPFE_N) return;
+    if (slot >= n || off + slot >= IPFE_N) continue;
     const int si = find_stream(chunk_prefix, nstreams, gc);
     const GaussStream *S = streams + si;
     const uint64_t v = vals[gc * 8 + slot];
@@ -373,12 +377,12 @@ __global__ void k_gauss_scatter(const GaussStream *__restrict__ streams, int nst
         if (neg) m = mod_prime_exact((uint32_t)(qmod(s) - m), s);
         out[s * IPFE_N] = m;
     }
+    }
 }
 
 __global__ void k_fill_i32(int *p, int v, size_t n)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 // sample_polya (src/sample.c:56-128): coefficient i = LE64(keystream[8i..8i+8))
@@ -420,8 +424,9 @@ int launch_gauss_candidates(const GaussStream *d_streams, int nstreams, const lo
                             long long ntiles, uint8_t *cnt, uint64_t *vals, int *status, hipStream_t st)
 {
     if (ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k_gauss_candidates, dim3((unsigned)ntiles), dim3(CAND_THREADS), 0, st, d_streams, nstreams,
-                       d_tile_prefix, g_aes_te, cnt, vals, status);
+    const unsigned grid = (unsigned)std::min<long long>(ntiles, 1ll << 20);
+    hipLaunchKernelGGL(k_gauss_candidates, dim3(grid), dim3(CAND_THREADS), 0, st, d_streams, nstreams, d_tile_prefix,
+                       ntiles, g_aes_te, cnt, vals, status);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -441,8 +446,9 @@ int launch_gauss_scatter(const GaussStream *d_streams, int nstreams, const long 
 {
     if (nchunks_total <= 0) return 0;
     const long long nthreads = nchunks_total * 8;
-    hipLaunchKernelGGL(k_gauss_scatter, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, d_streams,
-                       nstreams, d_chunk_prefix, nchunks_total, cnt, vals, pos);
+    const unsigned grid = (unsigned)std::min<long long>((nthreads + 255) / 256, 1ll << 20);
+    hipLaunchKernelGGL(k_gauss_scatter, dim3(grid), dim3(256), 0, st, d_streams, nstreams, d_chunk_prefix,
+                       nchunks_total, cnt, vals, pos);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -460,7 +466,8 @@ int launch_gauss_firsts(const GaussStream *d_streams, int nstreams, const long l
 int launch_fill_i32(int *p, int v, size_t n, hipStream_t st)
 {
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)1 << 20);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid), dim3(256), 0, st, p, v, n);
     HIP_OK(hipGetLastError());
     return 0;
 }

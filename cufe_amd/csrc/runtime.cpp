@@ -191,11 +191,15 @@ int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch
     size_t i = 0;
     while (i < reqs.size()) {
         // group streams so that the chunk scratch stays within budget
+        // and the chunk count stays below 2^28 (grid sizes, 32-bit work-item ids)
         size_t j = i, bytes = 0;
+        long long nch = 0;
         while (j < reqs.size()) {
             size_t b = gauss_bytes(reqs[j].which, reqs[j].ncalls, reqs[j].coef0_only);
-            if (j > i && bytes + b > scratch_budget) break;
+            long long c = gauss_window(reqs[j].which, reqs[j].ncalls, 0);
+            if (j > i && (bytes + b > scratch_budget || nch + c > (1ll << 28))) break;
             bytes += b;
+            nch += c;
             ++j;
         }
         int level = 0;

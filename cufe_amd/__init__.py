@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
         "cufe_sife_decrypt": (C.c_int, [_u32p, _u32p, _u32p, _u32p, C.c_int, C.c_int]),
         "cufe_time_ntt": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "cufe_time_gauss": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "cufe_microbench": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
         "rlwe_sife_setup_gpu": (None, [_u32p, _u32p, _u8p, _u8p, C.POINTER(C.c_float)]),
         "rlwe_sife_encrypt_gpu": (None, [_u32p, _u32p, _u32p, _u8p, _u8p, C.c_int, C.POINTER(C.c_float)]),
         "rlwe_sife_keygen_gpu": (None, [_u32p, _u32p, _u32p, C.c_int, C.POINTER(C.c_float)]),
@@ -195,6 +196,13 @@ def time_ntt(npoly: int, inverse: bool, iters: int) -> float:
     ms = C.c_float()
     _check(lib().cufe_time_ntt(npoly, int(inverse), iters, C.byref(ms)), "time_ntt")
     return ms.value
+
+
+def microbench(kind: int) -> float:
+    """0: fp64 FMA TFLOP/s, 1: int32 mul T ops/s, 2: copy GB/s, 3: AES G blocks/s."""
+    r = C.c_double()
+    _check(lib().cufe_microbench(kind, C.byref(r)), "microbench")
+    return r.value
 
 
 def time_gauss(nstreams: int, which: int, iters: int) -> float:

@@ -52,4 +52,53 @@ __device__ __forceinline__ void aes256_ctr_block(const uint32_t *__restrict__ rk
     o[3] = __builtin_bswap32(f3);
 }
 
+// Bank-replicated variant: T0 stored 32 times so that lane l reads copy
+// l mod 32 -- entry x of copy c at word 32 x + c, i.e. in bank c, so the 32
+// lanes of a half-wave never collide.  T1..T3 are rotations of T0.
+constexpr int AES_REP_WORDS = 256 * 32;
+
+__device__ __forceinline__ void aes_stage_tables_rep(uint32_t *lds_rep, const uint32_t *__restrict__ te_global)
+{
+    for (int i = threadIdx.x; i < AES_REP_WORDS; i += blockDim.x) lds_rep[i] = te_global[i >> 5];
+}
+
+__device__ __forceinline__ uint32_t ror32(uint32_t x, int n) { return __builtin_amdgcn_alignbit(x, x, n); }
+
+__device__ __forceinline__ void aes256_ctr_block_rep(const uint32_t *__restrict__ rk, const uint32_t *te_rep,
+                                                     uint32_t nonce, uint32_t ctr, uint32_t (&o)[4])
+{
+    const uint32_t *T = te_rep + (threadIdx.x & 31);
+#define TL(x) T[(x) << 5]
+    uint32_t s0 = (nonce << 16) ^ rk[0];
+    uint32_t s1 = rk[1];
+    uint32_t s2 = rk[2];
+    uint32_t s3 = ctr ^ rk[3];
+#pragma unroll
+    for (int r = 1; r < 14; ++r) {
+        uint32_t t0 = TL(s0 >> 24) ^ ror32(TL((s1 >> 16) & 255), 8) ^ ror32(TL((s2 >> 8) & 255), 16) ^
+                      ror32(TL(s3 & 255), 24) ^ rk[4 * r];
+        uint32_t t1 = TL(s1 >> 24) ^ ror32(TL((s2 >> 16) & 255), 8) ^ ror32(TL((s3 >> 8) & 255), 16) ^
+                      ror32(TL(s0 & 255), 24) ^ rk[4 * r + 1];
+        uint32_t t2 = TL(s2 >> 24) ^ ror32(TL((s3 >> 16) & 255), 8) ^ ror32(TL((s0 >> 8) & 255), 16) ^
+                      ror32(TL(s1 & 255), 24) ^ rk[4 * r + 2];
+        uint32_t t3 = TL(s3 >> 24) ^ ror32(TL((s0 >> 16) & 255), 8) ^ ror32(TL((s1 >> 8) & 255), 16) ^
+                      ror32(TL(s2 & 255), 24) ^ rk[4 * r + 3];
+        s0 = t0; s1 = t1; s2 = t2; s3 = t3;
+    }
+    // S-box byte of T0[x] is byte 1 (and byte 2)
+    uint32_t f0 = ((TL(s0 >> 24) << 16) & 0xff000000u) ^ (TL((s1 >> 16) & 255) & 0x00ff0000u) ^
+                  (TL((s2 >> 8) & 255) & 0x0000ff00u) ^ ((TL(s3 & 255) >> 8) & 0xffu) ^ rk[56];
+    uint32_t f1 = ((TL(s1 >> 24) << 16) & 0xff000000u) ^ (TL((s2 >> 16) & 255) & 0x00ff0000u) ^
+                  (TL((s3 >> 8) & 255) & 0x0000ff00u) ^ ((TL(s0 & 255) >> 8) & 0xffu) ^ rk[57];
+    uint32_t f2 = ((TL(s2 >> 24) << 16) & 0xff000000u) ^ (TL((s3 >> 16) & 255) & 0x00ff0000u) ^
+                  (TL((s0 >> 8) & 255) & 0x0000ff00u) ^ ((TL(s1 & 255) >> 8) & 0xffu) ^ rk[58];
+    uint32_t f3 = ((TL(s3 >> 24) << 16) & 0xff000000u) ^ (TL((s0 >> 16) & 255) & 0x00ff0000u) ^
+                  (TL((s1 >> 8) & 255) & 0x0000ff00u) ^ ((TL(s2 & 255) >> 8) & 0xffu) ^ rk[59];
+#undef TL
+    o[0] = __builtin_bswap32(f0);
+    o[1] = __builtin_bswap32(f1);
+    o[2] = __builtin_bswap32(f2);
+    o[3] = __builtin_bswap32(f3);
+}
+
 }  // namespace ipfe

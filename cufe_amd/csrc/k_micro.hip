@@ -1,0 +1,194 @@
+// k_micro.hip -- microbenchmarks that give the measured ceilings the roofline
+// fractions are read against: fp64 FMA rate, 32-bit integer multiply rate,
+// HBM copy bandwidth and raw AES-256-CTR keystream rate on this GPU.
+#include <stdio.h>
+
+#include "../../include/cufe_ipfe.h"
+#include "aes_dev.h"
+#include "runtime.h"
+
+namespace ipfe {
+
+// 16 independent FMA chains per thread; the result is stored so nothing is
+// optimised away.
+__global__ __launch_bounds__(256) void k_mb_fma64(double *out, int iters, double b, double c)
+{
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = (double)(threadIdx.x + i);
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = __fma_rn(acc[i], b, c);
+    }
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s += acc[i];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// 8 independent chains of (mul_lo, mul_hi, add): 3 int32 ops per step.
+__global__ __launch_bounds__(256) void k_mb_mul32(uint32_t *out, int iters, uint32_t b)
+{
+    uint32_t x[8], y[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        x[i] = threadIdx.x * 2654435761u + i;
+        y[i] = i;
+    }
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            uint32_t lo = x[i] * b;
+            uint32_t hi = __umulhi(x[i], b);
+            x[i] = lo + y[i];
+            y[i] = hi;
+        }
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s ^= x[i] ^ y[i];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ void k_mb_copy(const uint4 *__restrict__ in, uint4 *__restrict__ out, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+__global__ __launch_bounds__(256) void k_mb_aes(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global,
+                                                uint32_t nblocks_per_thread, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t te[AES_TABLE_WORDS];
+    aes_stage_tables(te, te_global);
+    __syncthreads();
+    uint32_t x = 0;
+    const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) * nblocks_per_thread;
+    for (uint32_t i = 0; i < nblocks_per_thread; ++i) {
+        uint32_t o[4];
+        aes256_ctr_block(rk, te, 7u, base + i, o);
+        x ^= o[0] ^ o[1] ^ o[2] ^ o[3];
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
+__global__ __launch_bounds__(256) void k_mb_aes_rep(const uint32_t *__restrict__ rk,
+                                                    const uint32_t *__restrict__ te_global, uint32_t nblocks_per_thread,
+                                                    uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t te[AES_REP_WORDS];
+    aes_stage_tables_rep(te, te_global);
+    __syncthreads();
+    uint32_t x = 0;
+    const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) * nblocks_per_thread;
+    for (uint32_t i = 0; i < nblocks_per_thread; ++i) {
+        uint32_t o[4];
+        aes256_ctr_block_rep(rk, te, 7u, base + i, o);
+        x ^= o[0] ^ o[1] ^ o[2] ^ o[3];
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
+// Checks that both AES variants produce the same keystream words.
+__global__ void k_mb_aes_check(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global,
+                               uint32_t *__restrict__ bad)
+{
+    __shared__ uint32_t te[AES_TABLE_WORDS];
+    __shared__ uint32_t rep[AES_REP_WORDS];
+    aes_stage_tables(te, te_global);
+    aes_stage_tables_rep(rep, te_global);
+    __syncthreads();
+    uint32_t o1[4], o2[4];
+    const uint32_t ctr = blockIdx.x * blockDim.x + threadIdx.x;
+    aes256_ctr_block(rk, te, 9u, ctr, o1);
+    aes256_ctr_block_rep(rk, rep, 9u, ctr, o2);
+    if (o1[0] != o2[0] || o1[1] != o2[1] || o1[2] != o2[2] || o1[3] != o2[3]) atomicAdd(bad, 1u);
+}
+
+}  // namespace ipfe
+
+using namespace ipfe;
+
+// kind 0: fp64 FMA (TFLOP/s), 1: int32 mul_lo+mul_hi+add (T int-ops/s),
+// 2: device copy (GB/s of read+write), 3: AES-256 CTR blocks (G blocks/s),
+// 4: the same with the bank-replicated table (and an equality check).
+extern "C" int cufe_microbench(int kind, double *result)
+{
+    Device *dev = device();
+    if (!dev) return -1;
+    hipStream_t st = dev->stream;
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+    const int blocks = cus * 8;
+    float ms = 0;
+    double work = 0;
+    if (kind == 0 || kind == 1 || kind == 3 || kind == 4) {
+        DevBuf<uint32_t> out;
+        if (out.alloc((size_t)blocks * 256 * 2)) return -1;
+        const int iters = 4096;
+        DevBuf<uint32_t> rk;
+        uint32_t hrk[60];
+        uint8_t key[32];
+        for (int i = 0; i < 32; ++i) key[i] = (uint8_t)i;
+        aes256_expand_key(key, hrk);
+        if (rk.alloc(60)) return -1;
+        HIP_OK(hipMemcpy(rk.get(), hrk, sizeof hrk, hipMemcpyHostToDevice));
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_OK(hipEventRecord(a, st));
+            if (kind == 0)
+                hipLaunchKernelGGL(k_mb_fma64, dim3(blocks), dim3(256), 0, st, (double *)out.get(), iters, 0.999999,
+                                   1e-9);
+            else if (kind == 1)
+                hipLaunchKernelGGL(k_mb_mul32, dim3(blocks), dim3(256), 0, st, out.get(), iters, 2654435761u);
+            else if (kind == 3)
+                hipLaunchKernelGGL(k_mb_aes, dim3(blocks), dim3(256), 0, st, rk.get(), g_aes_te, 64u, out.get());
+            else
+                hipLaunchKernelGGL(k_mb_aes_rep, dim3(blocks), dim3(256), 0, st, rk.get(), g_aes_te, 64u, out.get());
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipEventRecord(b, st));
+            HIP_OK(hipEventSynchronize(b));
+        }
+        HIP_OK(hipEventElapsedTime(&ms, a, b));
+        const double threads = (double)blocks * 256;
+        if (kind == 0) work = threads * iters * 16 * 2;       // FLOPs
+        else if (kind == 1) work = threads * iters * 8 * 3;   // int ops
+        else work = threads * 64;                             // AES blocks
+        if (kind == 4) {
+            DevBuf<uint32_t> bad;
+            uint32_t hb = 0;
+            if (bad.alloc(1)) return -1;
+            HIP_OK(hipMemset(bad.get(), 0, 4));
+            hipLaunchKernelGGL(k_mb_aes_check, dim3(64), dim3(256), 0, st, rk.get(), g_aes_te, bad.get());
+            HIP_OK(hipMemcpy(&hb, bad.get(), 4, hipMemcpyDeviceToHost));
+            if (hb) {
+                ipfe_set_error("bank-replicated AES disagrees with the T-table AES on %u blocks", hb);
+                return -1;
+            }
+        }
+    } else if (kind == 2) {
+        const size_t bytes = (size_t)1 << 30;
+        DevBuf<uint4> x, y;
+        if (x.alloc(bytes / 16) || y.alloc(bytes / 16)) return -1;
+        HIP_OK(hipMemset(x.get(), 1, bytes));
+        for (int rep = 0; rep < 3; ++rep) {
+            HIP_OK(hipEventRecord(a, st));
+            hipLaunchKernelGGL(k_mb_copy, dim3(cus * 16), dim3(256), 0, st, x.get(), y.get(), bytes / 16);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipEventRecord(b, st));
+            HIP_OK(hipEventSynchronize(b));
+        }
+        HIP_OK(hipEventElapsedTime(&ms, a, b));
+        work = 2.0 * bytes;
+    } else {
+        ipfe_set_error("cufe_microbench: unknown kind %d", kind);
+        return -1;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    const double s = ms * 1e-3;
+    *result = (kind == 0 || kind == 1) ? work / s / 1e12 : (kind == 2 ? work / s / 1e9 : work / s / 1e9);
+    return 0;
+}

@@ -188,6 +188,119 @@ __global__ __launch_bounds__(256) void k_contract(const int32_t *__restrict__ A,
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Contraction with the row operand in scalar registers.
+// Each wave owns 16 rows; their values at step k are wave-uniform, so they are
+// read with scalar loads (two s_load_dwordx16 per k-pair from a row store laid
+// out per 16-row block as [k][16] doubles) and enter v_fma_f64 as SGPR operands.
+// Each lane owns 4 keys read from LDS (two ds_read_b128 per k).  Block tile:
+// 64 rows (4 waves) x 256 keys; k-step 8 with the next B tile prefetched into
+// registers.  LDS traffic is 0.5 B per FMA, a quarter of the LDS-only tiling.
+// Tiles are ordered so that the blocks one XCD receives (block id mod 8) take
+// consecutive tiles: the key tiles of one row block share that XCD's L2.
+// ---------------------------------------------------------------------------
+constexpr int SW_ROWS = 16, SW_KEYS = 4, SBK = 8, SBN = 256, SBM = 64;
+
+__global__ __launch_bounds__(256) void k_contract_sgpr(const double *__restrict__ A, const double *__restrict__ B,
+                                                        int R, int S, int K, int lda, int ldb, uint32_t q, int kb,
+                                                        uint32_t *__restrict__ D, int ntn, int ntiles)
+{
+    __shared__ double Bs[2][SBK][SBN];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int per = gridDim.x / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (t >= ntiles) return;
+    const int tn = t % ntn, tm = t / ntn;
+    const int n0 = tn * SBN, m0 = tm * SBM + w * SW_ROWS;
+    const double qd = (double)q, qinv = 1.0 / (double)q;
+    const double *Ablk = A + (size_t)(m0 / SW_ROWS) * lda * SW_ROWS;
+    double acc[SW_ROWS][SW_KEYS];
+#pragma unroll
+    for (int i = 0; i < SW_ROWS; ++i)
+#pragma unroll
+        for (int j = 0; j < SW_KEYS; ++j) acc[i][j] = 0.0;
+    const int bkey = n0 + tid;
+    const bool bok = bkey < S;
+    const double *Bp = B + (size_t)(bok ? bkey : 0) * ldb;
+    double2 pre[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pre[u] = reinterpret_cast<const double2 *>(Bp)[u];
+    int buf = 0, steps = 0;
+    for (int k0 = 0; k0 < K; k0 += SBK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            Bs[buf][2 * u][tid] = bok ? pre[u].x : 0.0;
+            Bs[buf][2 * u + 1][tid] = bok ? pre[u].y : 0.0;
+        }
+        __syncthreads();
+        if (k0 + SBK < K) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pre[u] = reinterpret_cast<const double2 *>(Bp + k0 + SBK)[u];
+        }
+#pragma unroll 1
+        for (int kk = 0; kk < SBK; kk += 2) {
+            const double *ap = Ablk + (size_t)(k0 + kk) * SW_ROWS;
+            double a0[SW_ROWS], a1[SW_ROWS];
+#pragma unroll
+            for (int i = 0; i < SW_ROWS; ++i) {
+                a0[i] = ap[i];
+                a1[i] = ap[SW_ROWS + i];
+            }
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const double2 b01 = *reinterpret_cast<const double2 *>(&Bs[buf][kk + hh][lane * SW_KEYS]);
+                const double2 b23 = *reinterpret_cast<const double2 *>(&Bs[buf][kk + hh][lane * SW_KEYS + 2]);
+                const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+                for (int i = 0; i < SW_ROWS; ++i) {
+                    const double a = hh ? a1[i] : a0[i];
+#pragma unroll
+                    for (int j = 0; j < SW_KEYS; ++j) acc[i][j] = __fma_rn(a, bv[j], acc[i][j]);
+                }
+            }
+        }
+        buf ^= 1;
+        if (++steps == kb) {
+            steps = 0;
+#pragma unroll
+            for (int i = 0; i < SW_ROWS; ++i)
+#pragma unroll
+                for (int j = 0; j < SW_KEYS; ++j) acc[i][j] = reduce_bal(acc[i][j], qd, qinv);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < SW_ROWS; ++i) {
+        const int r = m0 + i;
+        if (r >= R) continue;
+#pragma unroll
+        for (int j = 0; j < SW_KEYS; ++j) {
+            const int v = n0 + lane * SW_KEYS + j;
+            if (v >= S) continue;
+            const double x = reduce_bal(acc[i][j], qd, qinv);
+            long long tt = (long long)x;
+            tt %= (long long)q;
+            if (tt < 0) tt += q;
+            D[(size_t)r * S + v] = (uint32_t)tt;
+        }
+    }
+}
+
+// int32 row store [3][cap][ldr] -> blocked fp64 store [3][cap64/16][ldr][16]
+// for rows [r0, r0 + R).
+__global__ void k_rows_blocked(const int32_t *__restrict__ a32, int cap, int cap64, int ldr, int r0, int R,
+                               double *__restrict__ a64)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)3 * R * ldr) return;
+    const int s = (int)(i / ((size_t)R * ldr));
+    const size_t rem = i % ((size_t)R * ldr);
+    const int r = r0 + (int)(rem / ldr), k = (int)(rem % ldr);
+    const size_t blk = (size_t)s * (cap64 / SW_ROWS) + r / SW_ROWS;
+    a64[(blk * ldr + k) * SW_ROWS + (r % SW_ROWS)] = (double)a32[((size_t)s * cap + r) * ldr + k];
+}
+
 // Exact integer contraction for key sets whose coefficients are too large for
 // exact fp64 partial sums: one thread per output, 64-bit products reduced one
 // by one.
@@ -372,6 +485,33 @@ int launch_contract(const int32_t *A, const double *B, int R, int S, int K, int 
     }
     dim3 grid((S + CT_BN - 1) / CT_BN, (R + CT_BM - 1) / CT_BM);
     hipLaunchKernelGGL(k_contract, grid, dim3(256), 0, st, A, B, R, S, K, lda, ldb, qmod(s), kb, D);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_contract_sgpr(const double *A, const double *B, int R, int S, int K, int lda, int ldb, int s, int kb,
+                         uint32_t *D, hipStream_t st)
+{
+    if (R <= 0 || S <= 0) return 0;
+    if (kb < 1 || K % SBK || ldb % 2 || K > lda || K > ldb) {
+        ipfe_set_error("contraction: bad shape K=%d lda=%d ldb=%d kb=%d", K, lda, ldb, kb);
+        return -1;
+    }
+    const int ntn = (S + SBN - 1) / SBN, ntm = (R + SBM - 1) / SBM;
+    const int ntiles = ntn * ntm;
+    const int grid = ((ntiles + 7) / 8) * 8;
+    hipLaunchKernelGGL(k_contract_sgpr, dim3(grid), dim3(256), 0, st, A, B, R, S, K, lda, ldb, qmod(s), kb, D, ntn,
+                       ntiles);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_rows_blocked(const int32_t *a32, int cap, int cap64, int ldr, int r0, int R, double *a64, hipStream_t st)
+{
+    const size_t n = (size_t)3 * R * ldr;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_rows_blocked, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a32, cap, cap64, ldr, r0, R,
+                       a64);
     HIP_OK(hipGetLastError());
     return 0;
 }

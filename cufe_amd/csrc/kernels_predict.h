@@ -25,6 +25,11 @@ int launch_contract_slow(const int32_t *A, const double *B, int R, int S, int K,
                          const BarrettConsts &bc, uint32_t *D, hipStream_t st);
 int launch_contract(const int32_t *A, const double *B, int R, int S, int K, int lda, int ldb, int s, int kb, uint32_t *D,
                     hipStream_t st);
+// Row operand in scalar registers: A is the blocked fp64 store (rows in groups
+// of 16, [k][16] per group, cap a multiple of 64), lda = its column count.
+int launch_contract_sgpr(const double *A, const double *B, int R, int S, int K, int lda, int ldb, int s, int kb,
+                         uint32_t *D, hipStream_t st);
+int launch_rows_blocked(const int32_t *a32, int cap, int cap64, int ldr, int r0, int R, double *a64, hipStream_t st);
 int launch_crt_kvalue(const uint32_t *D, size_t n, uint32_t *kval, hipStream_t st);
 int launch_kvalue(const uint32_t *D, const uint32_t *c0, int l, const SparseY &y, int R, int S, uint32_t *kval,
                   int *flag, hipStream_t st);

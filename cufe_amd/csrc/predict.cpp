@@ -10,6 +10,7 @@
 // each c_i are never materialised.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -45,6 +46,9 @@ struct cufe_ctx {
     int row_l = 0, cap = 0, nrows = 0;
     int ldr = 0;                  // row-store columns: N + row_l rounded up to 16
     DevBuf<int32_t> A;            // [3][cap][ldr]: c_L, then c_i[0] for i < row_l
+    DevBuf<double> A64;           // the same rows blocked for k_contract_sgpr: [3][(cap+64)/16][ldr][16]
+                                  // (64 pad rows per limb: a tile reads whole 64-row blocks)
+    int mode = 1;                 // contraction kernel: 1 = scalar-operand (sgpr), 0 = LDS tiles
     DevBuf<uint32_t> c0;          // [cap][row_l][3]
     // predict scratch
     int chunk_rows = 0;
@@ -252,7 +256,14 @@ extern "C" int cufe_ctx_rows_reserve(cufe_ctx *c, int cap, int l)
         return -1;
     }
     const int ldr = IPFE_N + ((l + 15) / 16) * 16;
-    if (c->A.alloc((size_t)3 * cap * ldr) || c->c0.alloc((size_t)cap * l * 3)) return -1;
+    cap = ((cap + 63) / 64) * 64;  // whole 64-row tiles of the blocked store
+    if (c->A.alloc((size_t)3 * cap * ldr) || c->c0.alloc((size_t)cap * l * 3) ||
+        c->A64.alloc((size_t)3 * (cap + 64) * ldr))
+        return -1;
+    HIP_OK(hipMemset(c->A.get(), 0, (size_t)3 * cap * ldr * 4));
+    HIP_OK(hipMemset(c->A64.get(), 0, (size_t)3 * (cap + 64) * ldr * 8));
+    const char *mode = getenv("CUFE_CONTRACT");
+    c->mode = (mode && strcmp(mode, "lds") == 0) ? 0 : 1;
     c->cap = cap;
     c->row_l = l;
     c->ldr = ldr;
@@ -329,6 +340,7 @@ extern "C" int cufe_ctx_encrypt_rows(cufe_ctx *c, const uint32_t *x, int n, cons
         if (launch_rows_c0(d_f0.get(), g, l, c->A.get(), c->cap, c->nrows + b0, c->ldr, st)) return -1;
         HIP_OK(hipMemcpyAsync(c->c0.get() + (size_t)(c->nrows + b0) * l * 3, d_f0.get(), (size_t)g * l * 3 * 4,
                               hipMemcpyDeviceToDevice, st));
+        if (launch_rows_blocked(c->A.get(), c->cap, c->cap + 64, c->ldr, c->nrows + b0, g, c->A64.get(), st)) return -1;
         if (sync_stream(st)) return -1;
     }
     HIP_OK(hipEventRecord(c->ev[7], st));
@@ -365,7 +377,9 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
     hipStream_t st = dev->stream;
     const int S = c->S, l = c->row_l;
     // rows per chunk: D (3 words) + kvalue (1 word) per (row, key) within ~4 GiB
-    const int chunk = (int)std::max<size_t>(128, std::min<size_t>((size_t)nrows, ((size_t)1 << 30) / ((size_t)S)));
+    // (a multiple of 64 rows, so every chunk starts on a blocked-store tile)
+    const int chunk = (int)std::max<size_t>(
+        64, std::min<size_t>(((size_t)nrows + 63) / 64 * 64, ((size_t)1 << 30) / ((size_t)S) / 64 * 64));
     if (c->D.n < (size_t)3 * chunk * S || c->labels.n < (size_t)chunk) {
         if (c->D.alloc((size_t)3 * chunk * S) || c->kval.alloc((size_t)chunk * S) || c->labels.alloc(chunk) ||
             c->flag.alloc(1))
@@ -382,9 +396,15 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
             const int32_t *As = c->A.get() + ((size_t)s * c->cap + r0) * c->ldr;
             const double *Bs = c->kd.get() + (size_t)s * S * c->kkey;
             uint32_t *Ds = c->D.get() + (size_t)s * R * S;
-            int rc = c->kb[s] >= 1 ? launch_contract(As, Bs, R, S, c->kkey, c->ldr, c->kkey, s, c->kb[s], Ds, st)
-                                   : launch_contract_slow(As, Bs, R, S, c->kkey, c->ldr, c->kkey, s, dev->barrett, Ds,
-                                                          st);
+            int rc;
+            if (c->kb[s] < 1)
+                rc = launch_contract_slow(As, Bs, R, S, c->kkey, c->ldr, c->kkey, s, dev->barrett, Ds, st);
+            else if (c->mode == 1 && r0 % 16 == 0)
+                // kb counts 16-product steps; this kernel's steps are 8 products
+                rc = launch_contract_sgpr(c->A64.get() + ((size_t)s * (c->cap + 64) + r0) * c->ldr, Bs, R, S, c->kkey,
+                                          c->ldr, c->kkey, s, 2 * c->kb[s], Ds, st);
+            else
+                rc = launch_contract(As, Bs, R, S, c->kkey, c->ldr, c->kkey, s, c->kb[s], Ds, st);
             if (rc) return -1;
         }
         HIP_OK(hipEventRecord(c->ev[1], st));

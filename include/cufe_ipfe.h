@@ -116,6 +116,11 @@ int cufe_aes256ctr(const unsigned char key[32], uint16_t nonce, uint32_t ctr0, u
 /* CRT + rounding of coefficient coef of nvec RNS polys d[nvec][3][4096]. */
 int cufe_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r);
 
+/* Measured ceilings of this GPU: kind 0 = fp64 FMA (TFLOP/s), 1 = 32-bit
+ * mul_lo + mul_hi + add (T int-ops/s), 2 = device copy (GB/s read + write),
+ * 3 = AES-256-CTR keystream (G blocks/s). */
+int cufe_microbench(int kind, double *result);
+
 /* The scheme at a vector length l <= 785 (uneven-length batching).  With
  * zero-padding, a length-l encryption equals the first l polynomials plus c_L
  * of the length-785 one (the noise order does not depend on x), so these are

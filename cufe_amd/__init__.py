@@ -257,6 +257,7 @@ def _ctx_sigs(lb):
         "cufe_ctx_model_info": (C.c_int, [p, C.POINTER(C.c_int), C.POINTER(C.c_uint)]),
         "cufe_ctx_get_model_keys": (C.c_int, [p, _u32p]),
         "cufe_ctx_get_row": (C.c_int, [p, C.c_int, _u32p, _u32p]),
+        "cufe_ctx_export_rows": (C.c_int, [p, C.c_int, C.c_int, _u32p, _u32p]),
         "cufe_svm_predict_gpu": (C.c_double, [C.POINTER(_SvmModelC), _u32p, _u32p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
@@ -350,6 +351,13 @@ class Context:
         cl = np.empty((NMODULI, N), np.uint32)
         c0 = np.empty((self.row_l, NMODULI), np.uint32)
         _check(lib().cufe_ctx_get_row(self._h, row, cl, c0), "ctx_get_row")
+        return cl, c0
+
+    def export_rows(self, row0: int, n: int):
+        """Compact ciphertexts of stored rows: c_L [n][3][N], c0 [n][l][3]."""
+        cl = np.empty((n, NMODULI, N), np.uint32)
+        c0 = np.empty((n, self.row_l, NMODULI), np.uint32)
+        _check(lib().cufe_ctx_export_rows(self._h, row0, n, cl, c0), "ctx_export_rows")
         return cl, c0
 
 

@@ -4,10 +4,9 @@
 // A context keeps in HBM: the master keys (mpk in the NTT domain, msk), the
 // functional keys of one SVM model (one per support vector, generated once on
 // the device instead of once per row as svm_predict_gpu does, main.c:308),
-// and a store of encrypted rows in compact form: c_L (balanced int32, the
-// operand of the coefficient-0 contraction) and coefficient 0 of c_0..c_{l-1}.
-// That is all decryption of coefficient 0 reads; the other 4095 coefficients of
-// each c_i are never materialised.
+// and a store of encrypted rows in compact form: c_L and coefficient 0 of
+// c_0..c_{l-1}.  That is all decryption of coefficient 0 reads; the other 4095
+// coefficients of each c_i are never materialised.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -16,9 +15,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "../../include/cufe_ipfe.h"
-#include "kernels_predict.h"
-#include "runtime.h"
+#include "resident.h"
 
 using namespace ipfe;
 
@@ -26,199 +23,77 @@ namespace {
 constexpr size_t kPoly = (size_t)3 * IPFE_N;
 }
 
-struct cufe_ctx {
-    int l_key = 0;                // slots of the master key
-    DevBuf<uint32_t> mpk, msk;    // [l_key+1][3][N], [l_key][3][N]
-    // model
-    int S = 0, nr_class = 0, sv_maxidx = -1;
-    DevBuf<int> y_off, y_idx;
-    DevBuf<uint32_t> y_val;
-    SparseY y{};
-    DevBuf<uint32_t> sk;          // [S][3][N]
-    DevBuf<double> kd;            // [3][S][kkey]
-    int kkey = 0;                 // key columns: N (+ the y_i slots when fused)
-    bool fused = false;           // sum part folded into the contraction
-    int kb[3] = {0, 0, 0};
-    unsigned int maxabs[3] = {0, 0, 0};
-    DevBuf<double> coef, rho;
-    SvmModelDev mdev{};
-    // rows
-    int row_l = 0, cap = 0, nrows = 0;
-    int ldr = 0;                  // row-store columns: N + row_l rounded up to 16
-    DevBuf<int32_t> A;            // [3][cap][ldr]: c_L, then c_i[0] for i < row_l
-    DevBuf<double> A64;           // the same rows blocked for k_contract_sgpr: [3][(cap+64)/16][ldr][16]
-                                  // (64 pad rows per limb: a tile reads whole 64-row blocks)
-    int mode = 1;                 // contraction kernel: 1 = scalar-operand (sgpr), 0 = LDS tiles
-    DevBuf<uint32_t> c0;          // [cap][row_l][3]
-    // predict scratch
-    int chunk_rows = 0;
-    DevBuf<uint32_t> D, kval;
-    DevBuf<double> labels;
-    DevBuf<int> flag;
-    hipEvent_t ev[8] = {};
-    float last_ms[4] = {0, 0, 0, 0};  // contract, kvalue, decide, total
-};
+namespace ipfe {
 
-static int ensure_events(cufe_ctx *c)
+int contract_mode()
 {
-    for (auto &e : c->ev)
-        if (!e) HIP_OK(hipEventCreate(&e));
-    return 0;
+    const char *m = getenv("CUFE_CONTRACT");
+    return (m && strcmp(m, "lds") == 0) ? 0 : 1;
 }
 
-extern "C" int cufe_ctx_create(cufe_ctx **out)
+// ---------------------------------------------------------------------------
+// Model
+// ---------------------------------------------------------------------------
+int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStream_t st)
 {
-    if (!out) {
-        ipfe_set_error("cufe_ctx_create: null output pointer");
+    if (!m || m->nr_class < 2 || m->nr_class > SVM_MAX_CLASS || m->l < 1) {
+        ipfe_set_error("model: nr_class %d (2..%d) or l %d unsupported", m ? m->nr_class : 0, SVM_MAX_CLASS,
+                       m ? m->l : 0);
         return -1;
     }
-    if (!device()) return -1;
-    cufe_ctx *c = new cufe_ctx();
-    if (ensure_events(c)) {
-        delete c;
-        return -1;
-    }
-    *out = c;
-    return 0;
-}
-
-extern "C" void cufe_ctx_destroy(cufe_ctx *c)
-{
-    if (!c) return;
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    delete c;
-}
-
-extern "C" int cufe_ctx_load_keys(cufe_ctx *c, const uint32_t *mpk, const uint32_t *msk, int l)
-{
-    if (!c || l < 1 || l > IPFE_L) {
-        ipfe_set_error("cufe_ctx_load_keys: bad context or length %d", l);
-        return -1;
-    }
-    Device *dev = device();
-    if (!dev) return -1;
-    if (c->mpk.alloc((size_t)(l + 1) * kPoly) || c->msk.alloc((size_t)l * kPoly)) return -1;
-    HIP_OK(hipMemcpy(c->mpk.get(), mpk, (size_t)(l + 1) * kPoly * 4, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(c->msk.get(), msk, (size_t)l * kPoly * 4, hipMemcpyHostToDevice));
-    c->l_key = l;
-    c->S = 0;
-    return 0;
-}
-
-// Generates resident keys on the device: a = sample_polya(seedA), then setup.
-extern "C" int cufe_ctx_setup(cufe_ctx *c, const unsigned char seedA[32], const unsigned char seed2[32],
-                              const unsigned char seed3[32], int l)
-{
-    if (!c || l < 1 || l > IPFE_L) {
-        ipfe_set_error("cufe_ctx_setup: bad context or length %d", l);
-        return -1;
-    }
-    Device *dev = device();
-    if (!dev) return -1;
-    hipStream_t st = dev->stream;
-    if (c->mpk.alloc((size_t)(l + 1) * kPoly) || c->msk.alloc((size_t)l * kPoly)) return -1;
-    DevBuf<uint32_t> d_rk, d_e, d_ap;
-    if (d_rk.alloc(60) || d_e.alloc((size_t)l * kPoly) || d_ap.alloc(kPoly)) return -1;
-    uint32_t rk[60];
-    aes256_expand_key(seedA, rk);
-    HIP_OK(hipMemcpyAsync(d_rk.get(), rk, sizeof rk, hipMemcpyHostToDevice, st));
-    uint32_t *ahat = c->mpk.get() + (size_t)l * kPoly;
-    if (launch_sample_polya(d_rk.get(), ahat, st)) return -1;
-    if (launch_ntt_fwd(ahat, ahat, 3, dev->tw_all, st)) return -1;
-    if (launch_shoup_prep(ahat, d_ap.get(), kPoly, st)) return -1;
-    std::vector<StreamReq> reqs;
-    reqs.push_back(make_stream(seed2, 0, 1, l, c->msk.get(), c->msk.get() + kPoly, (long long)kPoly));
-    reqs.push_back(make_stream(seed3, 0, 1, l, d_e.get(), d_e.get() + kPoly, (long long)kPoly));
-    if (run_gauss(reqs, st)) return -1;
-    if (launch_setup_pk(c->msk.get(), d_e.get(), ahat, d_ap.get(), c->mpk.get(), l, dev->tw_all, st)) return -1;
-    if (sync_stream(st)) return -1;
-    c->l_key = l;
-    c->S = 0;
-    return 0;
-}
-
-extern "C" int cufe_ctx_get_keys(cufe_ctx *c, uint32_t *mpk, uint32_t *msk)
-{
-    if (!c || c->l_key < 1) {
-        ipfe_set_error("cufe_ctx_get_keys: no keys in context");
-        return -1;
-    }
-    if (mpk) HIP_OK(hipMemcpy(mpk, c->mpk.get(), (size_t)(c->l_key + 1) * kPoly * 4, hipMemcpyDeviceToHost));
-    if (msk) HIP_OK(hipMemcpy(msk, c->msk.get(), (size_t)c->l_key * kPoly * 4, hipMemcpyDeviceToHost));
-    return 0;
-}
-
-// Functional keys of every support vector, generated on the device and kept.
-// The fold runs over the key length l_key (785 for the reference layout), so
-// it ends on zero slots exactly as the reference's dense loop does.
-extern "C" int cufe_ctx_set_model(cufe_ctx *c, const cufe_svm_model *m)
-{
-    if (!c || !m || c->l_key < 1) {
-        ipfe_set_error("cufe_ctx_set_model: context has no keys");
-        return -1;
-    }
-    if (m->nr_class < 2 || m->nr_class > SVM_MAX_CLASS || m->l < 1) {
-        ipfe_set_error("cufe_ctx_set_model: nr_class %d (2..%d) or l %d unsupported", m->nr_class, SVM_MAX_CLASS, m->l);
-        return -1;
-    }
-    Device *dev = device();
-    if (!dev) return -1;
-    hipStream_t st = dev->stream;
-    const int S = m->l, L = c->l_key;
-    std::vector<int> off(S + 1), idx;
+    const int nS = m->l;
+    std::vector<int> off(nS + 1), idx;
     std::vector<uint32_t> val;
-    int maxidx = -1;
-    for (int v = 0; v < S; ++v) {
+    int mx = -1;
+    for (int v = 0; v < nS; ++v) {
         off[v] = (int)idx.size();
         for (int i = 0; i < IPFE_L; ++i) {
             const uint32_t yv = m->sv[(size_t)v * IPFE_L + i];
             if (!yv) continue;
             if (i >= L) {
-                ipfe_set_error("cufe_ctx_set_model: SV %d has feature %d beyond the key length %d", v, i, L);
+                ipfe_set_error("model: SV %d has feature %d beyond the key length %d", v, i, L);
                 return -1;
             }
             idx.push_back(i);
             val.push_back(yv);
-            maxidx = std::max(maxidx, i);
+            mx = std::max(mx, i);
         }
     }
-    off[S] = (int)idx.size();
+    off[nS] = (int)idx.size();
     if (idx.empty()) {
         idx.push_back(0);
         val.push_back(0);
     }
     int nsv_sum = 0;
     for (int i = 0; i < m->nr_class; ++i) nsv_sum += m->nSV[i];
-    if (nsv_sum != S) {
-        ipfe_set_error("cufe_ctx_set_model: sum of nSV (%d) != l (%d)", nsv_sum, S);
+    if (nsv_sum != nS) {
+        ipfe_set_error("model: sum of nSV (%d) != l (%d)", nsv_sum, nS);
         return -1;
     }
     // With a zero slot after the last feature the reference's fold ends
     // canonical, so the sum part joins the contraction (key = [-s', y]).
-    const bool fused = maxidx < L - 1;
-    const int kkey = fused ? IPFE_N + ((maxidx + 1 + 15) / 16) * 16 : IPFE_N;
-    if (c->y_off.alloc(off.size()) || c->y_idx.alloc(idx.size()) || c->y_val.alloc(val.size()) ||
-        c->sk.alloc((size_t)S * kPoly) || c->kd.alloc((size_t)3 * S * kkey))
+    const bool fz = mx < L - 1;
+    const int kk = fz ? IPFE_N + ((mx + 1 + 15) / 16) * 16 : IPFE_N;
+    if (y_off.alloc(off.size()) || y_idx.alloc(idx.size()) || y_val.alloc(val.size()) ||
+        sk.alloc((size_t)nS * kPoly) || kd.alloc((size_t)3 * nS * kk))
         return -1;
-    HIP_OK(hipMemcpyAsync(c->y_off.get(), off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->y_idx.get(), idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->y_val.get(), val.data(), val.size() * 4, hipMemcpyHostToDevice, st));
-    c->y.off = c->y_off.get();
-    c->y.idx = c->y_idx.get();
-    c->y.val = c->y_val.get();
-    c->y.len = L;
-    if (launch_keygen(c->y, S, c->msk.get(), c->sk.get(), st)) return -1;
+    HIP_OK(hipMemcpyAsync(y_off.get(), off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(y_idx.get(), idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(y_val.get(), val.data(), val.size() * 4, hipMemcpyHostToDevice, st));
+    y.off = y_off.get();
+    y.idx = y_idx.get();
+    y.val = y_val.get();
+    y.len = L;
+    if (launch_keygen(y, nS, msk_dev, sk.get(), st)) return -1;
     DevBuf<unsigned int> d_max;
     if (d_max.alloc(3)) return -1;
-    HIP_OK(hipMemsetAsync(d_max.get(), 0, 12, st));
-    if (launch_contract_keys(c->sk.get(), S, kkey, fused ? -1 : 1, c->y, c->kd.get(), d_max.get(), st)) return -1;
-    HIP_OK(hipMemcpyAsync(c->maxabs, d_max.get(), 12, hipMemcpyDeviceToHost, st));
-    // model coefficients
+    if (launch_fill_i32((int *)d_max.get(), 0, 3, st)) return -1;
+    if (launch_contract_keys(sk.get(), nS, kk, fz ? -1 : 1, y, kd.get(), d_max.get(), st)) return -1;
+    HIP_OK(hipMemcpyAsync(maxabs, d_max.get(), 12, hipMemcpyDeviceToHost, st));
     const int npairs = m->nr_class * (m->nr_class - 1) / 2;
-    if (c->coef.alloc((size_t)(m->nr_class - 1) * S) || c->rho.alloc(npairs)) return -1;
-    HIP_OK(hipMemcpyAsync(c->coef.get(), m->sv_coef, sizeof(double) * (m->nr_class - 1) * S, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->rho.get(), m->rho, sizeof(double) * npairs, hipMemcpyHostToDevice, st));
+    if (coef.alloc((size_t)(m->nr_class - 1) * nS) || rho.alloc(npairs)) return -1;
+    HIP_OK(hipMemcpyAsync(coef.get(), m->sv_coef, sizeof(double) * (m->nr_class - 1) * nS, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(rho.get(), m->rho, sizeof(double) * npairs, hipMemcpyHostToDevice, st));
     if (sync_stream(st)) return -1;
     SvmModelDev md{};
     md.nr_class = m->nr_class;
@@ -229,70 +104,172 @@ extern "C" int cufe_ctx_set_model(cufe_ctx *c, const cufe_svm_model *m)
         md.label[i] = m->label[i];
         start += m->nSV[i];
     }
-    md.sv_coef = c->coef.get();
-    md.rho = c->rho.get();
-    c->mdev = md;
-    // reduction interval of the fp64 contraction: every partial sum below 2^53
+    md.sv_coef = coef.get();
+    md.rho = rho.get();
+    mdev = md;
+    // reduction interval (16-product steps) of the fp64 contraction: every
+    // partial sum stays an exact integer below 2^53
     for (int s = 0; s < 3; ++s) {
         const double q = (double)qmod(s);
-        const double bmax = (double)c->maxabs[s];
+        const double bmax = (double)maxabs[s];
         const double room = 9007199254740991.0 - 1.5 * q - 1.0;
-        double kb = bmax > 0 ? floor(room / (16.0 * floor(q / 2) * bmax)) : 256.0;
-        c->kb[s] = (int)std::min(256.0, kb);
+        double k = bmax > 0 ? floor(room / (16.0 * floor(q / 2) * bmax)) : 256.0;
+        kb[s] = (int)std::min(256.0, k);
     }
-    c->S = S;
-    c->nr_class = m->nr_class;
-    c->sv_maxidx = maxidx;
-    c->kkey = kkey;
-    c->fused = fused;
+    S = nS;
+    nr_class = m->nr_class;
+    maxidx = mx;
+    l_key = L;
+    kkey = kk;
+    fused = fz;
     return 0;
 }
 
-// Reserves the row store: up to cap rows of l slots (l must cover every SV feature).
-extern "C" int cufe_ctx_rows_reserve(cufe_ctx *c, int cap, int l)
+size_t Model::bytes() const { return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8); }
+
+// ---------------------------------------------------------------------------
+// RowStore
+// ---------------------------------------------------------------------------
+size_t RowStore::bytes_per_row(int l)
 {
-    if (!c || cap < 1 || l < 1 || l > c->l_key) {
-        ipfe_set_error("cufe_ctx_rows_reserve: cap %d / l %d invalid for key length %d", cap, l, c ? c->l_key : 0);
+    const size_t ldr = IPFE_N + ((l + 15) / 16) * 16;
+    return 3 * ldr * (4 + 8) + (size_t)l * 12;
+}
+
+int RowStore::reserve(int cap_rows, int len)
+{
+    if (cap_rows < 1 || len < 1 || len > IPFE_L) {
+        ipfe_set_error("row store: cap %d / l %d invalid", cap_rows, len);
         return -1;
     }
-    const int ldr = IPFE_N + ((l + 15) / 16) * 16;
-    cap = ((cap + 63) / 64) * 64;  // whole 64-row tiles of the blocked store
-    if (c->A.alloc((size_t)3 * cap * ldr) || c->c0.alloc((size_t)cap * l * 3) ||
-        c->A64.alloc((size_t)3 * (cap + 64) * ldr))
+    const int ld = IPFE_N + ((len + 15) / 16) * 16;
+    const int cp = ((cap_rows + 63) / 64) * 64;  // whole 64-row tiles of the blocked store
+    if (A.alloc((size_t)3 * cp * ld) || c0.alloc((size_t)cp * len * 3) || A64.alloc((size_t)3 * (cp + 64) * ld))
         return -1;
-    HIP_OK(hipMemset(c->A.get(), 0, (size_t)3 * cap * ldr * 4));
-    HIP_OK(hipMemset(c->A64.get(), 0, (size_t)3 * (cap + 64) * ldr * 8));
-    const char *mode = getenv("CUFE_CONTRACT");
-    c->mode = (mode && strcmp(mode, "lds") == 0) ? 0 : 1;
-    c->cap = cap;
-    c->row_l = l;
-    c->ldr = ldr;
-    c->nrows = 0;
+    HIP_OK(hipMemset(A.get(), 0, (size_t)3 * cp * ld * 4));
+    HIP_OK(hipMemset(A64.get(), 0, (size_t)3 * (cp + 64) * ld * 8));
+    cap = cp;
+    l = len;
+    ldr = ld;
     return 0;
 }
 
-// Encrypts n rows x [n][row_l] into the row store (compact form).  Row b uses
-// nonce nonce0 + b, exactly as rlwe_sife_encrypt_gpu numbers its batch.
-extern "C" int cufe_ctx_encrypt_rows(cufe_ctx *c, const uint32_t *x, int n, const unsigned char seed2[32],
-                                     const unsigned char seed3[32], int nonce0, float *ms)
+int RowStore::put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev, hipStream_t st)
 {
-    if (!c || c->cap < 1 || c->l_key < 1) {
-        ipfe_set_error("cufe_ctx_encrypt_rows: reserve rows and load keys first");
+    if (r0 < 0 || n < 0 || r0 + n > cap) {
+        ipfe_set_error("row store: rows [%d, %d) exceed capacity %d", r0, r0 + n, cap);
         return -1;
     }
-    if (n < 0 || c->nrows + n > c->cap) {
-        ipfe_set_error("cufe_ctx_encrypt_rows: %d rows exceed the reserved capacity %d", c->nrows + n, c->cap);
+    if (n == 0) return 0;
+    if (launch_rows_balanced(cl_dev, kPoly, n, A.get(), cap, r0, ldr, st)) return -1;
+    if (launch_rows_c0(c0_dev, n, l, A.get(), cap, r0, ldr, st)) return -1;
+    HIP_OK(hipMemcpyAsync(c0.get() + (size_t)r0 * l * 3, c0_dev, (size_t)n * l * 3 * 4, hipMemcpyDeviceToDevice, st));
+    if (launch_rows_blocked(A.get(), cap, cap + 64, ldr, r0, n, A64.get(), st)) return -1;
+    return 0;
+}
+
+int Scratch::reserve(int rows, int S)
+{
+    if (D.n < (size_t)3 * rows * S || kval.n < (size_t)rows * S || labels.n < (size_t)rows) {
+        if (D.alloc((size_t)3 * rows * S) || kval.alloc((size_t)rows * S) || labels.alloc(rows)) return -1;
+    }
+    if (!flag.get() && flag.alloc(1)) return -1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Prediction
+// ---------------------------------------------------------------------------
+int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratch &sc, int scratch_rows, int mode,
+                 double *labels_host, uint32_t *kval_host, PredictTimers *tm, hipStream_t st)
+{
+    if (m.maxidx >= rs.l) {
+        ipfe_set_error("predict: rows hold %d slots but the model uses feature %d", rs.l, m.maxidx);
         return -1;
     }
-    if (nonce0 < 0 || nonce0 + n > 65536) {
-        ipfe_set_error("cufe_ctx_encrypt_rows: nonce range [%d, %d) exceeds 16 bits", nonce0, nonce0 + n);
+    if (m.kkey > rs.ldr) {
+        ipfe_set_error("predict: key width %d exceeds row width %d", m.kkey, rs.ldr);
         return -1;
     }
     Device *dev = device();
     if (!dev) return -1;
-    hipStream_t st = dev->stream;
-    const int l = c->row_l;
-    HIP_OK(hipEventRecord(c->ev[6], st));
+    const int S = m.S;
+    const int chunk = std::max(64, scratch_rows / 64 * 64);
+    if (sc.reserve(std::min(chunk, ((nrows + 63) / 64) * 64), S)) return -1;
+    if (tm) tm->ms[0] = tm->ms[1] = tm->ms[2] = 0;
+    if (launch_fill_i32(sc.flag.get(), 0, 1, st)) return -1;
+    for (int r0 = row0; r0 < row0 + nrows; r0 += chunk) {
+        const int R = std::min(chunk, row0 + nrows - r0);
+        if (tm) HIP_OK(hipEventRecord(tm->ev[0], st));
+        for (int s = 0; s < 3; ++s) {
+            const int32_t *As = rs.A.get() + ((size_t)s * rs.cap + r0) * rs.ldr;
+            const double *Bs = m.kd.get() + (size_t)s * S * m.kkey;
+            uint32_t *Ds = sc.D.get() + (size_t)s * R * S;
+            int rc;
+            if (m.kb[s] < 1)
+                rc = launch_contract_slow(As, Bs, R, S, m.kkey, rs.ldr, m.kkey, s, dev->barrett, Ds, st);
+            else if (mode == 1 && r0 % 16 == 0)
+                // kb counts 16-product steps; this kernel's steps are 8 products
+                rc = launch_contract_sgpr(rs.A64.get() + ((size_t)s * (rs.cap + 64) + r0) * rs.ldr, Bs, R, S, m.kkey,
+                                          rs.ldr, m.kkey, s, 2 * m.kb[s], Ds, st);
+            else
+                rc = launch_contract(As, Bs, R, S, m.kkey, rs.ldr, m.kkey, s, m.kb[s], Ds, st);
+            if (rc) return -1;
+        }
+        if (tm) HIP_OK(hipEventRecord(tm->ev[1], st));
+        if (m.fused) {
+            if (launch_crt_kvalue(sc.D.get(), (size_t)R * S, sc.kval.get(), st)) return -1;
+        } else if (launch_kvalue(sc.D.get(), rs.c0.get() + (size_t)r0 * rs.l * 3, rs.l, m.y, R, S, sc.kval.get(),
+                                 sc.flag.get(), st)) {
+            return -1;
+        }
+        if (tm) HIP_OK(hipEventRecord(tm->ev[2], st));
+        if (launch_decide(sc.kval.get(), R, S, m.mdev, sc.labels.get(), st)) return -1;
+        if (tm) HIP_OK(hipEventRecord(tm->ev[3], st));
+        if (labels_host)
+            HIP_OK(hipMemcpyAsync(labels_host + (r0 - row0), sc.labels.get(), sizeof(double) * R,
+                                  hipMemcpyDeviceToHost, st));
+        if (kval_host)
+            HIP_OK(hipMemcpyAsync(kval_host + (size_t)(r0 - row0) * S, sc.kval.get(), sizeof(uint32_t) * R * S,
+                                  hipMemcpyDeviceToHost, st));
+        if (tm || r0 + chunk < row0 + nrows) {
+            // the scratch is reused by the next chunk: wait for this one
+            HIP_OK(hipStreamSynchronize(st));
+            if (tm) {
+                float a = 0, b = 0, d = 0;
+                HIP_OK(hipEventElapsedTime(&a, tm->ev[0], tm->ev[1]));
+                HIP_OK(hipEventElapsedTime(&b, tm->ev[1], tm->ev[2]));
+                HIP_OK(hipEventElapsedTime(&d, tm->ev[2], tm->ev[3]));
+                tm->ms[0] += a;
+                tm->ms[1] += b;
+                tm->ms[2] += d;
+            }
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Compact encryption
+// ---------------------------------------------------------------------------
+int encrypt_rows_compact(const uint32_t *mpk_dev, int l_key, RowStore &rs, int r0, const uint32_t *x, int n,
+                         const unsigned char seed2[32], const unsigned char seed3[32], int nonce0, hipStream_t st)
+{
+    if (n < 0 || r0 < 0 || r0 + n > rs.cap) {
+        ipfe_set_error("encrypt: rows [%d, %d) exceed the reserved capacity %d", r0, r0 + n, rs.cap);
+        return -1;
+    }
+    if (nonce0 < 0 || nonce0 + n > 65536) {
+        ipfe_set_error("encrypt: nonce range [%d, %d) exceeds 16 bits", nonce0, nonce0 + n);
+        return -1;
+    }
+    if (rs.l > l_key) {
+        ipfe_set_error("encrypt: rows of %d slots exceed the key length %d", rs.l, l_key);
+        return -1;
+    }
+    Device *dev = device();
+    if (!dev) return -1;
+    const int l = rs.l;
     // rows per group: the S2 stream (r, f_0) keeps per-chunk values, the compact
     // S3 stream only counts
     const size_t per_row = gauss_bytes(2, 2, 0) + gauss_bytes(3, l, 1) + 4 * kPoly * 4 + (size_t)l * 16;
@@ -330,19 +307,161 @@ extern "C" int cufe_ctx_encrypt_rows(cufe_ctx *c, const uint32_t *x, int n, cons
         HIP_OK(hipMemcpyAsync(d_msgs.get(), msgs.data(), sizeof(EncMsg) * g, hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync(d_items.get(), items.data(), sizeof(EncItem) * g, hipMemcpyHostToDevice, st));
         // c_L = INTT(a_hat . r_hat) + f_0 against the key's a_hat at index l_key
-        if (launch_enc_poly(d_items.get(), g, d_msgs.get(), c->mpk.get(), c->l_key, dev->tw_all, dev->ninv, dev->scale,
-                            st))
+        if (launch_enc_poly(d_items.get(), g, d_msgs.get(), mpk_dev, l_key, dev->tw_all, dev->ninv, dev->scale, st))
             return -1;
-        if (launch_enc_coef0(d_r.get(), c->mpk.get(), d_x.get(), g, l, d_f0.get(), dev->ninv, dev->scale, dev->barrett,
-                             st))
+        if (launch_enc_coef0(d_r.get(), mpk_dev, d_x.get(), g, l, d_f0.get(), dev->ninv, dev->scale, dev->barrett, st))
             return -1;
-        if (launch_rows_balanced(d_cl.get(), kPoly, g, c->A.get(), c->cap, c->nrows + b0, c->ldr, st)) return -1;
-        if (launch_rows_c0(d_f0.get(), g, l, c->A.get(), c->cap, c->nrows + b0, c->ldr, st)) return -1;
-        HIP_OK(hipMemcpyAsync(c->c0.get() + (size_t)(c->nrows + b0) * l * 3, d_f0.get(), (size_t)g * l * 3 * 4,
-                              hipMemcpyDeviceToDevice, st));
-        if (launch_rows_blocked(c->A.get(), c->cap, c->cap + 64, c->ldr, c->nrows + b0, g, c->A64.get(), st)) return -1;
+        if (rs.put(r0 + b0, g, d_cl.get(), d_f0.get(), st)) return -1;
         if (sync_stream(st)) return -1;
     }
+    return 0;
+}
+
+}  // namespace ipfe
+
+// ===========================================================================
+// Context API
+// ===========================================================================
+struct cufe_ctx {
+    int l_key = 0;              // slots of the master key
+    DevBuf<uint32_t> mpk, msk;  // [l_key+1][3][N], [l_key][3][N]
+    Model model;
+    RowStore rows;
+    int nrows = 0;
+    Scratch scratch;
+    hipEvent_t ev[8] = {};
+};
+
+namespace ipfe {
+const uint32_t *ctx_msk(const cufe_ctx *c) { return c->msk.get(); }
+const uint32_t *ctx_mpk(const cufe_ctx *c) { return c->mpk.get(); }
+int ctx_key_len(const cufe_ctx *c) { return c->l_key; }
+}  // namespace ipfe
+
+extern "C" int cufe_ctx_create(cufe_ctx **out)
+{
+    if (!out) {
+        ipfe_set_error("cufe_ctx_create: null output pointer");
+        return -1;
+    }
+    if (!device()) return -1;
+    cufe_ctx *c = new cufe_ctx();
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            ipfe_set_error("cufe_ctx_create: hipEventCreate failed");
+            delete c;
+            return -1;
+        }
+    *out = c;
+    return 0;
+}
+
+extern "C" void cufe_ctx_destroy(cufe_ctx *c)
+{
+    if (!c) return;
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete c;
+}
+
+extern "C" int cufe_ctx_load_keys(cufe_ctx *c, const uint32_t *mpk, const uint32_t *msk, int l)
+{
+    if (!c || l < 1 || l > IPFE_L) {
+        ipfe_set_error("cufe_ctx_load_keys: bad context or length %d", l);
+        return -1;
+    }
+    if (!device()) return -1;
+    if (c->mpk.alloc((size_t)(l + 1) * kPoly) || c->msk.alloc((size_t)l * kPoly)) return -1;
+    HIP_OK(hipMemcpy(c->mpk.get(), mpk, (size_t)(l + 1) * kPoly * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->msk.get(), msk, (size_t)l * kPoly * 4, hipMemcpyHostToDevice));
+    c->l_key = l;
+    c->model.S = 0;
+    return 0;
+}
+
+// Generates resident keys on the device: a = sample_polya(seedA), then setup.
+extern "C" int cufe_ctx_setup(cufe_ctx *c, const unsigned char seedA[32], const unsigned char seed2[32],
+                              const unsigned char seed3[32], int l)
+{
+    if (!c || l < 1 || l > IPFE_L) {
+        ipfe_set_error("cufe_ctx_setup: bad context or length %d", l);
+        return -1;
+    }
+    Device *dev = device();
+    if (!dev) return -1;
+    hipStream_t st = dev->stream;
+    if (c->mpk.alloc((size_t)(l + 1) * kPoly) || c->msk.alloc((size_t)l * kPoly)) return -1;
+    DevBuf<uint32_t> d_rk, d_e, d_ap;
+    if (d_rk.alloc(60) || d_e.alloc((size_t)l * kPoly) || d_ap.alloc(kPoly)) return -1;
+    uint32_t rk[60];
+    aes256_expand_key(seedA, rk);
+    HIP_OK(hipMemcpyAsync(d_rk.get(), rk, sizeof rk, hipMemcpyHostToDevice, st));
+    uint32_t *ahat = c->mpk.get() + (size_t)l * kPoly;
+    if (launch_sample_polya(d_rk.get(), ahat, st)) return -1;
+    if (launch_ntt_fwd(ahat, ahat, 3, dev->tw_all, st)) return -1;
+    if (launch_shoup_prep(ahat, d_ap.get(), kPoly, st)) return -1;
+    std::vector<StreamReq> reqs;
+    reqs.push_back(make_stream(seed2, 0, 1, l, c->msk.get(), c->msk.get() + kPoly, (long long)kPoly));
+    reqs.push_back(make_stream(seed3, 0, 1, l, d_e.get(), d_e.get() + kPoly, (long long)kPoly));
+    if (run_gauss(reqs, st)) return -1;
+    if (launch_setup_pk(c->msk.get(), d_e.get(), ahat, d_ap.get(), c->mpk.get(), l, dev->tw_all, st)) return -1;
+    if (sync_stream(st)) return -1;
+    c->l_key = l;
+    c->model.S = 0;
+    return 0;
+}
+
+extern "C" int cufe_ctx_get_keys(cufe_ctx *c, uint32_t *mpk, uint32_t *msk)
+{
+    if (!c || c->l_key < 1) {
+        ipfe_set_error("cufe_ctx_get_keys: no keys in context");
+        return -1;
+    }
+    if (mpk) HIP_OK(hipMemcpy(mpk, c->mpk.get(), (size_t)(c->l_key + 1) * kPoly * 4, hipMemcpyDeviceToHost));
+    if (msk) HIP_OK(hipMemcpy(msk, c->msk.get(), (size_t)c->l_key * kPoly * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Functional keys of every support vector, generated on the device and kept.
+// The fold runs over the key length l_key (785 for the reference layout), so
+// it ends on zero slots exactly as the reference's dense loop does.
+extern "C" int cufe_ctx_set_model(cufe_ctx *c, const cufe_svm_model *m)
+{
+    if (!c || !m || c->l_key < 1) {
+        ipfe_set_error("cufe_ctx_set_model: context has no keys");
+        return -1;
+    }
+    Device *dev = device();
+    if (!dev) return -1;
+    return c->model.build(c->msk.get(), c->l_key, m, dev->stream);
+}
+
+// Reserves the row store: up to cap rows of l slots (l must cover every SV feature).
+extern "C" int cufe_ctx_rows_reserve(cufe_ctx *c, int cap, int l)
+{
+    if (!c || l > c->l_key) {
+        ipfe_set_error("cufe_ctx_rows_reserve: l %d invalid for key length %d", l, c ? c->l_key : 0);
+        return -1;
+    }
+    if (c->rows.reserve(cap, l)) return -1;
+    c->nrows = 0;
+    return 0;
+}
+
+// Encrypts n rows x [n][row_l] into the row store (compact form).  Row b uses
+// nonce nonce0 + b, exactly as rlwe_sife_encrypt_gpu numbers its batch.
+extern "C" int cufe_ctx_encrypt_rows(cufe_ctx *c, const uint32_t *x, int n, const unsigned char seed2[32],
+                                     const unsigned char seed3[32], int nonce0, float *ms)
+{
+    if (!c || c->rows.cap < 1 || c->l_key < 1) {
+        ipfe_set_error("cufe_ctx_encrypt_rows: reserve rows and load keys first");
+        return -1;
+    }
+    Device *dev = device();
+    if (!dev) return -1;
+    hipStream_t st = dev->stream;
+    HIP_OK(hipEventRecord(c->ev[6], st));
+    if (encrypt_rows_compact(c->mpk.get(), c->l_key, c->rows, c->nrows, x, n, seed2, seed3, nonce0, st)) return -1;
     HIP_OK(hipEventRecord(c->ev[7], st));
     HIP_OK(hipEventSynchronize(c->ev[7]));
     float e = 0;
@@ -360,7 +479,7 @@ extern "C" int cufe_ctx_rows_count(const cufe_ctx *c) { return c ? c->nrows : -1
 // device milliseconds of this call.
 extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels, uint32_t *kvalues, float *ms)
 {
-    if (!c || c->S < 1) {
+    if (!c || c->model.S < 1) {
         ipfe_set_error("cufe_ctx_predict: no model in context");
         return -1;
     }
@@ -368,80 +487,28 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
         ipfe_set_error("cufe_ctx_predict: rows [%d, %d) outside the %d stored", row0, row0 + nrows, c->nrows);
         return -1;
     }
-    if (c->sv_maxidx >= c->row_l) {
-        ipfe_set_error("cufe_ctx_predict: rows hold %d slots but the model uses feature %d", c->row_l, c->sv_maxidx);
-        return -1;
-    }
     Device *dev = device();
     if (!dev) return -1;
     hipStream_t st = dev->stream;
-    const int S = c->S, l = c->row_l;
-    // rows per chunk: D (3 words) + kvalue (1 word) per (row, key) within ~4 GiB
-    // (a multiple of 64 rows, so every chunk starts on a blocked-store tile)
-    const int chunk = (int)std::max<size_t>(
-        64, std::min<size_t>(((size_t)nrows + 63) / 64 * 64, ((size_t)1 << 30) / ((size_t)S) / 64 * 64));
-    if (c->D.n < (size_t)3 * chunk * S || c->labels.n < (size_t)chunk) {
-        if (c->D.alloc((size_t)3 * chunk * S) || c->kval.alloc((size_t)chunk * S) || c->labels.alloc(chunk) ||
-            c->flag.alloc(1))
-            return -1;
-        c->chunk_rows = chunk;
-    }
-    float t_contract = 0, t_kv = 0, t_dec = 0;
+    // rows per chunk: D (3 words) + kvalue (1 word) per (row, key) within ~16 GiB
+    const int chunk_rows = (int)std::max<size_t>(64, ((size_t)1 << 30) / (size_t)c->model.S);
+    PredictTimers tm;
+    for (int i = 0; i < 4; ++i) tm.ev[i] = c->ev[i];
     HIP_OK(hipEventRecord(c->ev[4], st));
-    if (launch_fill_i32(c->flag.get(), 0, 1, st)) return -1;
-    for (int r0 = row0; r0 < row0 + nrows; r0 += chunk) {
-        const int R = std::min(chunk, row0 + nrows - r0);
-        HIP_OK(hipEventRecord(c->ev[0], st));
-        for (int s = 0; s < 3; ++s) {
-            const int32_t *As = c->A.get() + ((size_t)s * c->cap + r0) * c->ldr;
-            const double *Bs = c->kd.get() + (size_t)s * S * c->kkey;
-            uint32_t *Ds = c->D.get() + (size_t)s * R * S;
-            int rc;
-            if (c->kb[s] < 1)
-                rc = launch_contract_slow(As, Bs, R, S, c->kkey, c->ldr, c->kkey, s, dev->barrett, Ds, st);
-            else if (c->mode == 1 && r0 % 16 == 0)
-                // kb counts 16-product steps; this kernel's steps are 8 products
-                rc = launch_contract_sgpr(c->A64.get() + ((size_t)s * (c->cap + 64) + r0) * c->ldr, Bs, R, S, c->kkey,
-                                          c->ldr, c->kkey, s, 2 * c->kb[s], Ds, st);
-            else
-                rc = launch_contract(As, Bs, R, S, c->kkey, c->ldr, c->kkey, s, c->kb[s], Ds, st);
-            if (rc) return -1;
-        }
-        HIP_OK(hipEventRecord(c->ev[1], st));
-        if (c->fused) {
-            if (launch_crt_kvalue(c->D.get(), (size_t)R * S, c->kval.get(), st)) return -1;
-        } else if (launch_kvalue(c->D.get(), c->c0.get() + (size_t)r0 * l * 3, l, c->y, R, S, c->kval.get(),
-                                 c->flag.get(), st)) {
-            return -1;
-        }
-        HIP_OK(hipEventRecord(c->ev[2], st));
-        if (launch_decide(c->kval.get(), R, S, c->mdev, c->labels.get(), st)) return -1;
-        HIP_OK(hipEventRecord(c->ev[3], st));
-        if (labels) HIP_OK(hipMemcpyAsync(labels + (r0 - row0), c->labels.get(), sizeof(double) * R,
-                                          hipMemcpyDeviceToHost, st));
-        if (kvalues)
-            HIP_OK(hipMemcpyAsync(kvalues + (size_t)(r0 - row0) * S, c->kval.get(), sizeof(uint32_t) * R * S,
-                                  hipMemcpyDeviceToHost, st));
-        HIP_OK(hipEventSynchronize(c->ev[3]));
-        float a = 0, b = 0, d = 0;
-        HIP_OK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-        HIP_OK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-        HIP_OK(hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
-        t_contract += a;
-        t_kv += b;
-        t_dec += d;
-    }
+    if (predict_rows(c->model, c->rows, row0, nrows, c->scratch, chunk_rows, contract_mode(), labels, kvalues, &tm, st))
+        return -1;
     int flag = 0;
-    HIP_OK(hipMemcpyAsync(&flag, c->flag.get(), 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&flag, c->scratch.flag.get(), 4, hipMemcpyDeviceToHost, st));
     HIP_OK(hipEventRecord(c->ev[5], st));
     HIP_OK(hipEventSynchronize(c->ev[5]));
     float tot = 0;
     HIP_OK(hipEventElapsedTime(&tot, c->ev[4], c->ev[5]));
-    c->last_ms[0] = t_contract;
-    c->last_ms[1] = t_kv;
-    c->last_ms[2] = t_dec;
-    c->last_ms[3] = tot;
-    if (ms) memcpy(ms, c->last_ms, sizeof c->last_ms);
+    if (ms) {
+        ms[0] = tm.ms[0];
+        ms[1] = tm.ms[1];
+        ms[2] = tm.ms[2];
+        ms[3] = tot;
+    }
     if (flag) {
         ipfe_set_error("cufe_ctx_predict: a decryption sum ended on the representative q with a zero product; "
                        "this needs the full-polynomial path (keys must be shorter than the slot count)");
@@ -454,13 +521,13 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
 // balanced key coefficient per limb.
 extern "C" int cufe_ctx_model_info(const cufe_ctx *c, int *kb, unsigned int *maxabs)
 {
-    if (!c || c->S < 1) {
+    if (!c || c->model.S < 1) {
         ipfe_set_error("cufe_ctx_model_info: no model");
         return -1;
     }
     for (int s = 0; s < 3; ++s) {
-        if (kb) kb[s] = c->kb[s];
-        if (maxabs) maxabs[s] = c->maxabs[s];
+        if (kb) kb[s] = c->model.kb[s];
+        if (maxabs) maxabs[s] = c->model.maxabs[s];
     }
     return 0;
 }
@@ -468,31 +535,38 @@ extern "C" int cufe_ctx_model_info(const cufe_ctx *c, int *kb, unsigned int *max
 // The resident functional keys [S][3][N] (tests).
 extern "C" int cufe_ctx_get_model_keys(const cufe_ctx *c, uint32_t *sk)
 {
-    if (!c || c->S < 1) {
+    if (!c || c->model.S < 1) {
         ipfe_set_error("cufe_ctx_get_model_keys: no model");
         return -1;
     }
-    HIP_OK(hipMemcpy(sk, c->sk.get(), (size_t)c->S * kPoly * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(sk, c->model.sk.get(), (size_t)c->model.S * kPoly * 4, hipMemcpyDeviceToHost));
     return 0;
 }
 
-// The stored compact ciphertext of one row: c_L [3][N] canonical and
-// coefficient 0 of c_0..c_{l-1} as [l][3] (tests).
-extern "C" int cufe_ctx_get_row(const cufe_ctx *c, int row, uint32_t *cl, uint32_t *c0)
+// Stored compact ciphertexts of rows [row0, row0+n): c_L [n][3][N] canonical
+// and coefficient 0 of c_0..c_{l-1} as [n][l][3].
+extern "C" int cufe_ctx_export_rows(const cufe_ctx *c, int row0, int n, uint32_t *cl, uint32_t *c0)
 {
-    if (!c || row < 0 || row >= c->nrows) {
-        ipfe_set_error("cufe_ctx_get_row: row %d not stored", row);
+    if (!c || row0 < 0 || n < 0 || row0 + n > c->nrows) {
+        ipfe_set_error("cufe_ctx_export_rows: rows [%d, %d) not stored", row0, row0 + n);
         return -1;
     }
-    std::vector<int32_t> a(kPoly);
-    for (int s = 0; s < 3; ++s)
-        HIP_OK(hipMemcpy(a.data() + (size_t)s * IPFE_N, c->A.get() + ((size_t)s * c->cap + row) * c->ldr,
-                         IPFE_N * 4, hipMemcpyDeviceToHost));
-    for (int s = 0; s < 3; ++s)
-        for (int j = 0; j < IPFE_N; ++j) {
-            int64_t v = a[(size_t)s * IPFE_N + j];
-            cl[(size_t)s * IPFE_N + j] = (uint32_t)(v < 0 ? v + qmod(s) : v);
+    const RowStore &rs = c->rows;
+    std::vector<int32_t> a((size_t)n * rs.ldr);
+    for (int s = 0; s < 3; ++s) {
+        HIP_OK(hipMemcpy(a.data(), rs.A.get() + ((size_t)s * rs.cap + row0) * rs.ldr, (size_t)n * rs.ldr * 4,
+                         hipMemcpyDeviceToHost));
+        for (int r = 0; r < n; ++r) {
+            uint32_t *dst = cl + ((size_t)r * 3 + s) * IPFE_N;
+            const int32_t *src = a.data() + (size_t)r * rs.ldr;
+            for (int j = 0; j < IPFE_N; ++j) dst[j] = (uint32_t)(src[j] < 0 ? (int64_t)src[j] + qmod(s) : src[j]);
         }
-    HIP_OK(hipMemcpy(c0, c->c0.get() + (size_t)row * c->row_l * 3, (size_t)c->row_l * 3 * 4, hipMemcpyDeviceToHost));
+    }
+    HIP_OK(hipMemcpy(c0, rs.c0.get() + (size_t)row0 * rs.l * 3, (size_t)n * rs.l * 3 * 4, hipMemcpyDeviceToHost));
     return 0;
+}
+
+extern "C" int cufe_ctx_get_row(const cufe_ctx *c, int row, uint32_t *cl, uint32_t *c0)
+{
+    return cufe_ctx_export_rows(c, row, 1, cl, c0);
 }

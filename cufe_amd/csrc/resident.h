@@ -1,0 +1,78 @@
+// resident.h -- device-resident objects of the batched prediction path:
+// functional keys of one SVM model, a store of compact encrypted rows, and the
+// scratch of one prediction pass.  Used by the context API (predict.cpp) and
+// the serving loop (serve.cpp).
+#pragma once
+#include <stdint.h>
+
+#include "../../include/cufe_ipfe.h"
+#include "kernels_predict.h"
+#include "runtime.h"
+
+namespace ipfe {
+
+// Functional keys of one linear SVM model, generated on the device from the
+// master secret key (rlwe_sife_keygen over the key length, src/rlwe_sife.c:215-245),
+// plus the contraction operand [3][S][kkey] and the vote parameters.
+struct Model {
+    int S = 0, nr_class = 0, maxidx = -1, l_key = 0;
+    int kkey = 0;       // key columns: N, plus the y slots when fused
+    bool fused = false; // sum part of decryption folded into the contraction
+    int kb[3] = {0, 0, 0};
+    unsigned int maxabs[3] = {0, 0, 0};
+    DevBuf<int> y_off, y_idx;
+    DevBuf<uint32_t> y_val;
+    SparseY y{};
+    DevBuf<uint32_t> sk;  // [S][3][N]
+    DevBuf<double> kd;    // [3][S][kkey]
+    DevBuf<double> coef, rho;
+    SvmModelDev mdev{};
+
+    int build(const uint32_t *msk_dev, int l_key, const cufe_svm_model *m, hipStream_t st);
+    size_t bytes() const;
+};
+
+// Compact encrypted rows of l slots: c_L and coefficient 0 of c_0..c_{l-1},
+// kept as [3][cap][ldr] int32 (balanced; ldr = N + l rounded up to 16), the
+// same rows blocked for the scalar-operand contraction ([3][(cap+64)/16][ldr][16]
+// fp64), and [cap][l][3] canonical coefficient-0 residues.
+struct RowStore {
+    int l = 0, cap = 0, ldr = 0;
+    DevBuf<int32_t> A;
+    DevBuf<double> A64;
+    DevBuf<uint32_t> c0;
+
+    int reserve(int cap, int l);
+    // rows [r0, r0+n) from device buffers: cl [n][3][N] canonical, c0 [n][l][3]
+    int put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev, hipStream_t st);
+    static size_t bytes_per_row(int l);
+};
+
+struct Scratch {
+    DevBuf<uint32_t> D, kval;
+    DevBuf<double> labels;
+    DevBuf<int> flag;
+    int reserve(int rows, int S);
+};
+
+// Predicts rows [r0, r0+R) of the store against the model: labels and
+// (optional) kvalues copied to host memory asynchronously on st.  Chunked so
+// that the scratch stays within `scratch_rows` rows.  ms (optional, 3 floats,
+// only when timing events are passed): contraction / key-value / vote time.
+struct PredictTimers {
+    hipEvent_t ev[4];
+    float ms[3];
+};
+int predict_rows(const Model &m, const RowStore &rs, int r0, int R, Scratch &sc, int scratch_rows, int mode,
+                 double *labels_host, uint32_t *kval_host, PredictTimers *tm, hipStream_t st);
+
+// Encrypts n rows x [n][l] (row b with nonce nonce0 + b) into store rows
+// [r0, r0+n), against master keys mpk [l_key+1][3][N] on the device.
+int encrypt_rows_compact(const uint32_t *mpk_dev, int l_key, RowStore &rs, int r0, const uint32_t *x, int n,
+                         const unsigned char seed2[32], const unsigned char seed3[32], int nonce0, hipStream_t st);
+
+// Contraction kernel choice: 1 = scalar-operand (default), 0 = LDS tiles
+// (CUFE_CONTRACT=lds).
+int contract_mode();
+
+}  // namespace ipfe

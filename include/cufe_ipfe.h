@@ -180,6 +180,9 @@ int cufe_ctx_model_info(const cufe_ctx *ctx, int *kb, unsigned int *maxabs);
  * coefficient 0 of c_0..c_{l-1} as [l][3]), for inspection. */
 int cufe_ctx_get_model_keys(const cufe_ctx *ctx, uint32_t *sk);
 int cufe_ctx_get_row(const cufe_ctx *ctx, int row, uint32_t *cl, uint32_t *c0);
+/* Stored rows [row0, row0+n) in compact wire form: c_L [n][3][4096] and
+ * coefficient 0 of c_0..c_{l-1} as [n][l][3] (what a serving request carries). */
+int cufe_ctx_export_rows(const cufe_ctx *ctx, int row0, int n, uint32_t *cl, uint32_t *c0);
 
 #ifdef __cplusplus
 }

@@ -361,6 +361,190 @@ class Context:
         return cl, c0
 
 
+def _encbatch_sigs(lb):
+    p = C.c_void_p
+    sig = {
+        "cufe_encbatch_create": (C.c_int, [p, _i32p, C.c_int, C.POINTER(C.c_void_p)]),
+        "cufe_encbatch_run": (C.c_int, [p, _u32p, _u8p, _u8p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "cufe_encbatch_export": (C.c_int, [p, C.c_int, C.c_int, _u32p]),
+        "cufe_encbatch_checksum": (C.c_int, [p, C.POINTER(C.c_uint64), C.POINTER(C.c_longlong)]),
+        "cufe_encbatch_destroy": (None, [p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lb, name)
+        f.restype = res
+        f.argtypes = args
+
+
+class EncBatch:
+    """Batched encryption of messages of uneven lengths against a Context's keys;
+    ciphertexts stay on the GPU (export() copies some back)."""
+
+    def __init__(self, ctx: "Context", lens):
+        lb = lib()
+        if not getattr(lb, "_encbatch_sigs", False):
+            _encbatch_sigs(lb)
+            lb._encbatch_sigs = True
+        self.lens = np.ascontiguousarray(lens, np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(self.lens)]).astype(np.int64)
+        h = C.c_void_p()
+        _check(lb.cufe_encbatch_create(ctx._h, self.lens, int(self.lens.size), C.byref(h)), "encbatch_create")
+        self._h = h
+        self._ctx = ctx
+
+    def run(self, x_packed, seed2, seed3, nonce0: int = 0, use_graph: bool = False) -> float:
+        x = _u32(x_packed)
+        assert x.size == self.offsets[-1]
+        ms = C.c_float()
+        _check(lib().cufe_encbatch_run(self._h, x, _seed(seed2), _seed(seed3), nonce0, int(use_graph), C.byref(ms)),
+               "encbatch_run")
+        return ms.value
+
+    def export(self, b0: int, nb: int):
+        """List of ciphertexts [l_b+1][3][N] for messages b0..b0+nb-1."""
+        ls = self.lens[b0:b0 + nb]
+        words = int(np.sum(ls.astype(np.int64) + 1)) * NMODULI * N
+        flat = np.empty(words, np.uint32)
+        _check(lib().cufe_encbatch_export(self._h, b0, nb, flat), "encbatch_export")
+        out, o = [], 0
+        for l in ls:
+            k = (int(l) + 1) * NMODULI * N
+            out.append(flat[o:o + k].reshape(int(l) + 1, NMODULI, N))
+            o += k
+        return out
+
+    def checksum(self):
+        s = C.c_uint64()
+        info = (C.c_longlong * 4)()
+        _check(lib().cufe_encbatch_checksum(self._h, C.byref(s), info), "encbatch_checksum")
+        return s.value, dict(zip(("messages", "words", "groups", "redone"), list(info)))
+
+    def close(self):
+        if self._h:
+            lib().cufe_encbatch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _server_sigs(lb):
+    p = C.c_void_p
+    pp = C.POINTER(C.c_void_p)
+    sig = {
+        "cufe_server_create": (C.c_int, [p, C.c_size_t, C.c_int, C.c_int, pp]),
+        "cufe_server_add_model": (C.c_int, [p, C.POINTER(_SvmModelC), C.POINTER(C.c_int)]),
+        "cufe_server_start": (C.c_int, [p]),
+        "cufe_server_submit": (C.c_int, [p, C.c_int, C.c_int, C.c_int, _u32p, _u32p, _f64p, C.c_void_p,
+                                         C.POINTER(C.c_uint64)]),
+        "cufe_server_wait": (C.c_int, [p, C.c_uint64, C.c_double, C.POINTER(C.c_double)]),
+        "cufe_server_replay": (C.c_int, [p, C.c_int, _f64p, _i32p, _i32p, _i32p, pp, pp, pp, _f64p,
+                                         C.POINTER(C.c_double)]),
+        "cufe_server_stats": (C.c_int, [p, C.POINTER(C.c_longlong)]),
+        "cufe_server_destroy": (None, [p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lb, name)
+        f.restype = res
+        f.argtypes = args
+
+
+class Server:
+    """Continuous-batching prediction server over the keys of a Context.
+
+    Requests carry rows in compact ciphertext form (Context.export_rows):
+    c_L [n][3][N] and coefficient 0 of c_0..c_{l-1} as [n][l][3]."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_batch_rows: int = 0, batching: bool = True):
+        lb = lib()
+        if not getattr(lb, "_server_sigs", False):
+            _server_sigs(lb)
+            lb._server_sigs = True
+        h = C.c_void_p()
+        _check(lb.cufe_server_create(ctx._h, hbm_budget, max_batch_rows, int(bool(batching)), C.byref(h)),
+               "server_create")
+        self._h = h
+        self._ctx = ctx
+        self._models = []
+        self._structs = []
+        self._inflight = {}
+
+    def add_model(self, model: SvmModel) -> int:
+        st = model.as_struct()
+        mid = C.c_int()
+        _check(lib().cufe_server_add_model(self._h, C.byref(st), C.byref(mid)), "server_add_model")
+        self._models.append(model)
+        self._structs.append(st)
+        return mid.value
+
+    def start(self):
+        _check(lib().cufe_server_start(self._h), "server_start")
+
+    def submit(self, model: int, cl, c0, want_kvalues: bool = False) -> int:
+        cl = _u32(cl)
+        c0 = _u32(c0)
+        n, l = c0.shape[0], c0.shape[1]
+        assert cl.shape == (n, NMODULI, N) and c0.shape == (n, l, NMODULI)
+        labels = np.empty(n, np.float64)
+        kv = np.empty((n, self._models[model].l), np.uint32) if want_kvalues else None
+        t = C.c_uint64()
+        _check(lib().cufe_server_submit(self._h, model, n, l, cl, c0, labels,
+                                        kv.ctypes.data if kv is not None else None, C.byref(t)), "server_submit")
+        self._inflight[t.value] = (cl, c0, labels, kv)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0):
+        """(labels, kvalues, latency_ms), or None while pending; raises on failure."""
+        lat = C.c_double()
+        rc = lib().cufe_server_wait(self._h, ticket, timeout_ms, C.byref(lat))
+        if rc == 0:
+            return None
+        bufs = self._inflight.pop(ticket)
+        if rc < 0:
+            raise CufeError(f"server_wait: {lib().cufe_last_error().decode()}")
+        return bufs[2], bufs[3], lat.value
+
+    def replay(self, arrival_s, model, rows):
+        """Open-loop replay; rows[i] = (cl, c0) of request i.  Returns
+        (labels list, latency_ms array, makespan_s, failed count)."""
+        n = len(arrival_s)
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        mdl = np.ascontiguousarray(model, np.int32)
+        keep = [(_u32(cl), _u32(c0)) for cl, c0 in rows]
+        nr = np.array([c0.shape[0] for _, c0 in keep], np.int32)
+        ls = np.array([c0.shape[1] for _, c0 in keep], np.int32)
+        labels = [np.empty(k, np.float64) for k in nr]
+        clp = (C.c_void_p * n)(*[a.ctypes.data for a, _ in keep])
+        c0p = (C.c_void_p * n)(*[b.ctypes.data for _, b in keep])
+        lbp = (C.c_void_p * n)(*[x.ctypes.data for x in labels])
+        lat = np.empty(n, np.float64)
+        mk = C.c_double()
+        failed = lib().cufe_server_replay(self._h, n, arr, mdl, nr, ls, clp, c0p, lbp, lat, C.byref(mk))
+        if failed < 0:
+            raise CufeError(f"server_replay: {lib().cufe_last_error().decode()}")
+        return labels, lat, mk.value, failed
+
+    def stats(self) -> dict:
+        out = (C.c_longlong * 8)()
+        _check(lib().cufe_server_stats(self._h, out), "server_stats")
+        keys = ("requests", "rows", "batches", "failed", "max_batch_rows", "cap_rows", "model_bytes", "slot_bytes")
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        if self._h:
+            lib().cufe_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def svm_predict_plain(model: SvmModel, c_full, msk) -> tuple:
     """svm_predict_gpu (main.c:270-390) through the reference ABI on one full
     ciphertext c [786][3][N] with msk [785][3][N]: (label, kvalues)."""

@@ -6,6 +6,8 @@
 // with y_i = 0 only maps acc = q to 0, so the kernels visit the non-zero y_i of
 // a sparse list and canonicalise acc wherever zero slots were skipped; the
 // result is identical to the dense loop.
+#include <algorithm>
+
 #include "crt_dev.h"
 #include "kernels.h"
 
@@ -93,6 +95,34 @@ int launch_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r, hipStre
 {
     if (nvec <= 0) return 0;
     hipLaunchKernelGGL(k_crt_round, dim3((nvec + 255) / 256), dim3(256), 0, st, d, nvec, coef, r);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// Position-weighted checksum of a word array: sum of (2i+1) * w_i (mod 2^64),
+// so a moved, dropped or changed word changes it.  Grid-stride; one 64-bit
+// atomic add per workgroup.
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t *__restrict__ w, size_t n,
+                                                  unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long part[256];
+    unsigned long long acc = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc += (2ull * i + 1ull) * (unsigned long long)w[i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
+int launch_checksum(const uint32_t *w, size_t n, unsigned long long *out, hipStream_t st)
+{
+    if (n == 0) return 0;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_checksum, dim3((unsigned)blocks), dim3(256), 0, st, w, n, out);
     HIP_OK(hipGetLastError());
     return 0;
 }

@@ -101,5 +101,7 @@ int launch_dec_sum(const SparseY &y, int nvec, const uint32_t *c, uint32_t *d, h
 int launch_poly_sub(uint32_t *d, const uint32_t *prod, size_t n_total, hipStream_t st);
 // CRT + rounding of coefficient k of d[v][3][N] -> r[v]  (k = 0 for kvalues).
 int launch_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r, hipStream_t st);
+// out += sum over i of (2i+1) * w[i] mod 2^64 (out must be cleared first)
+int launch_checksum(const uint32_t *w, size_t n, unsigned long long *out, hipStream_t st);
 
 }  // namespace ipfe

@@ -154,16 +154,18 @@ int RowStore::reserve(int cap_rows, int len)
     return 0;
 }
 
-int RowStore::put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev, hipStream_t st)
+int RowStore::put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev, hipStream_t st, int src_l)
 {
-    if (r0 < 0 || n < 0 || r0 + n > cap) {
-        ipfe_set_error("row store: rows [%d, %d) exceed capacity %d", r0, r0 + n, cap);
+    if (src_l < 0) src_l = l;
+    if (r0 < 0 || n < 0 || r0 + n > cap || src_l < 1 || src_l > l) {
+        ipfe_set_error("row store: rows [%d, %d) of %d slots do not fit capacity %d x %d", r0, r0 + n, src_l, cap, l);
         return -1;
     }
     if (n == 0) return 0;
     if (launch_rows_balanced(cl_dev, kPoly, n, A.get(), cap, r0, ldr, st)) return -1;
-    if (launch_rows_c0(c0_dev, n, l, A.get(), cap, r0, ldr, st)) return -1;
-    HIP_OK(hipMemcpyAsync(c0.get() + (size_t)r0 * l * 3, c0_dev, (size_t)n * l * 3 * 4, hipMemcpyDeviceToDevice, st));
+    if (launch_rows_c0(c0_dev, n, src_l, A.get(), cap, r0, ldr, st)) return -1;
+    HIP_OK(hipMemcpy2DAsync(c0.get() + (size_t)r0 * l * 3, (size_t)l * 12, c0_dev, (size_t)src_l * 12,
+                            (size_t)src_l * 12, n, hipMemcpyDeviceToDevice, st));
     if (launch_rows_blocked(A.get(), cap, cap + 64, ldr, r0, n, A64.get(), st)) return -1;
     return 0;
 }

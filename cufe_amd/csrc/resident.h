@@ -43,8 +43,9 @@ struct RowStore {
     DevBuf<uint32_t> c0;
 
     int reserve(int cap, int l);
-    // rows [r0, r0+n) from device buffers: cl [n][3][N] canonical, c0 [n][l][3]
-    int put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev, hipStream_t st);
+    // rows [r0, r0+n) from device buffers: cl [n][3][N] canonical, c0
+    // [n][src_l][3] with src_l <= l slots (default l); slots past src_l read 0
+    int put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev, hipStream_t st, int src_l = -1);
     static size_t bytes_per_row(int l);
 };
 

@@ -115,21 +115,24 @@ size_t gauss_bytes(int which, int ncalls, int coef0_only)
     return coef0_only ? (size_t)ch * 1 + (size_t)ncalls * 8 : (size_t)ch * (1 + 64 + 4);
 }
 
-static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t st, int *overflow)
+int GaussPlan::build(const StreamReq *reqs, int count, int lvl)
 {
     constexpr int kTile = 64;
-    std::vector<GaussStream> hs(n);
-    std::vector<long long> tile_prefix(n), chunk_prefix(n), first_prefix(n);
-    long long tiles = 0, chunks = 0, full_chunks = 0, nfirst = 0;
+    n = count;
+    level = lvl;
+    hs.assign(n, GaussStream{});
+    tile_prefix.assign(n, 0);
+    chunk_prefix.assign(n, 0);
+    first_prefix.assign(n, 0);
+    tiles = chunks = full_chunks = nfirst = 0;
     // full streams first, so per-chunk values and labels cover only their chunks
     std::vector<int> order;
     for (int i = 0; i < n; ++i)
         if (!reqs[i].coef0_only) order.push_back(i);
     for (int i = 0; i < n; ++i)
         if (reqs[i].coef0_only) order.push_back(i);
-    for (int k = 0; k < n; ++k) {
-        const int i = k;
-        const StreamReq &rq = reqs[order[k]];
+    for (int i = 0; i < n; ++i) {
+        const StreamReq &rq = reqs[order[i]];
         GaussStream &g = hs[i];
         memcpy(g.rk, rq.rk, sizeof g.rk);
         g.nonce = rq.nonce;
@@ -151,29 +154,69 @@ static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t 
         if (g.coef0_only) nfirst += g.ncalls;
         else full_chunks = chunks;
     }
-    // per-chunk values and labels are only kept up to the last full stream
-    DevBuf<GaussStream> d_streams;
-    DevBuf<long long> d_tp, d_cp, d_fp, d_first;
-    DevBuf<uint8_t> d_cnt;
-    DevBuf<uint64_t> d_vals;
-    DevBuf<int> d_pos, d_status;
-    if (d_streams.alloc(n) || d_tp.alloc(n) || d_cp.alloc(n) || d_fp.alloc(n) || d_cnt.alloc(chunks) ||
-        d_vals.alloc((size_t)full_chunks * 8 + 1) || d_pos.alloc((size_t)full_chunks + 1) ||
-        d_first.alloc((size_t)nfirst + 1) || d_status.alloc(n + 1))
+    if (d_streams.alloc(n) || d_tp.alloc(n) || d_cp.alloc(n) || d_fp.alloc(n) || d_status.alloc(n + 1) ||
+        d_first.alloc((size_t)nfirst + 1))
         return -1;
+    return 0;
+}
+
+size_t GaussPlan::scratch_bytes() const { return (size_t)chunks + (size_t)full_chunks * (64 + 4) + 64; }
+
+int GaussPlan::bind_scratch(uint8_t *cnt, uint64_t *vals, int *pos)
+{
+    s_cnt = cnt;
+    s_vals = vals;
+    s_pos = pos;
+    return 0;
+}
+
+int GaussPlan::alloc_scratch()
+{
+    if (own_cnt.alloc((size_t)chunks) || own_vals.alloc((size_t)full_chunks * 8 + 1) ||
+        own_pos.alloc((size_t)full_chunks + 1))
+        return -1;
+    return bind_scratch(own_cnt.get(), own_vals.get(), own_pos.get());
+}
+
+// Re-keys the streams in place (same shapes, new round keys / nonces).
+void GaussPlan::rekey(const StreamReq *reqs)
+{
+    std::vector<int> order;
+    for (int i = 0; i < n; ++i)
+        if (!reqs[i].coef0_only) order.push_back(i);
+    for (int i = 0; i < n; ++i)
+        if (reqs[i].coef0_only) order.push_back(i);
+    for (int i = 0; i < n; ++i) {
+        memcpy(hs[i].rk, reqs[order[i]].rk, sizeof hs[i].rk);
+        hs[i].nonce = reqs[order[i]].nonce;
+    }
+}
+
+int GaussPlan::upload(hipStream_t st)
+{
     HIP_OK(hipMemcpyAsync(d_streams.get(), hs.data(), sizeof(GaussStream) * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(d_tp.get(), tile_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(d_cp.get(), chunk_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(d_fp.get(), first_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
-    if (launch_fill_i32(d_pos.get(), -1, (size_t)full_chunks, st) || launch_fill_i32(d_status.get(), 0, n + 1, st))
+    return 0;
+}
+
+// Kernels only (fills, candidates, walk, scatter, firsts): capturable.
+int GaussPlan::launch(hipStream_t st)
+{
+    if (launch_fill_i32(s_pos, -1, (size_t)full_chunks, st) || launch_fill_i32(d_status.get(), 0, n + 1, st))
         return -1;
-    if (launch_gauss_candidates(d_streams.get(), n, d_tp.get(), tiles, d_cnt.get(), d_vals.get(), d_status.get() + n,
-                                st))
+    if (launch_gauss_candidates(d_streams.get(), n, d_tp.get(), tiles, s_cnt, s_vals, d_status.get() + n, st))
         return -1;
-    if (launch_gauss_walk(d_streams.get(), n, d_cnt.get(), d_pos.get(), d_first.get(), d_status.get(), st)) return -1;
-    if (launch_gauss_scatter(d_streams.get(), n, d_cp.get(), full_chunks, d_cnt.get(), d_vals.get(), d_pos.get(), st))
+    if (launch_gauss_walk(d_streams.get(), n, s_cnt, s_pos, d_first.get(), d_status.get(), st)) return -1;
+    if (launch_gauss_scatter(d_streams.get(), n, d_cp.get(), full_chunks, s_cnt, s_vals, s_pos, st)) return -1;
+    if (launch_gauss_firsts(d_streams.get(), n, d_fp.get(), d_first.get(), nfirst, d_status.get() + n, st))
         return -1;
-    if (launch_gauss_firsts(d_streams.get(), n, d_fp.get(), d_first.get(), nfirst, d_status.get() + n, st)) return -1;
+    return 0;
+}
+
+int GaussPlan::check(hipStream_t st, int *overflow)
+{
     std::vector<int> status(n + 1);
     HIP_OK(hipMemcpyAsync(status.data(), d_status.get(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -184,6 +227,13 @@ static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t 
     *overflow = 0;
     for (int i = 0; i < n; ++i) *overflow |= status[i];
     return 0;
+}
+
+static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t st, int *overflow)
+{
+    GaussPlan p;
+    if (p.build(reqs, n, level) || p.alloc_scratch() || p.upload(st) || p.launch(st)) return -1;
+    return p.check(st, overflow);
 }
 
 int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch_budget)

@@ -73,6 +73,36 @@ StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncal
 // sequential sampler; splits into groups that fit scratch_budget bytes.
 int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch_budget = (size_t)8 << 30);
 
+// One group of sampler streams, split into host planning, table upload, the
+// kernel launches (no host round trip, so a graph can capture them) and the
+// status check that reports a speculative window that was too short.
+struct GaussPlan {
+    int n = 0, level = 0;
+    long long tiles = 0, chunks = 0, full_chunks = 0, nfirst = 0;
+    std::vector<GaussStream> hs;
+    std::vector<long long> tile_prefix, chunk_prefix, first_prefix;
+    DevBuf<GaussStream> d_streams;
+    DevBuf<long long> d_tp, d_cp, d_fp, d_first;
+    DevBuf<int> d_status;
+    DevBuf<uint8_t> own_cnt;
+    DevBuf<uint64_t> own_vals;
+    DevBuf<int> own_pos;
+    uint8_t *s_cnt = nullptr;
+    uint64_t *s_vals = nullptr;
+    int *s_pos = nullptr;
+
+    int build(const StreamReq *reqs, int n, int level);
+    // chunk scratch: own allocation, or a caller's buffer of scratch_bytes()
+    // (chunks counts, then 8 values and 1 position per full-stream chunk)
+    size_t scratch_bytes() const;
+    int alloc_scratch();
+    int bind_scratch(uint8_t *cnt, uint64_t *vals, int *pos);
+    void rekey(const StreamReq *reqs);
+    int upload(hipStream_t st);
+    int launch(hipStream_t st);
+    int check(hipStream_t st, int *overflow);
+};
+
 // Speculative window (chunks) for a stream of ncalls calls of sampler `which`
 // at margin level m, and the scratch bytes one stream needs at level 0.
 long long gauss_window(int which, int ncalls, int m);

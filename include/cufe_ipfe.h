@@ -19,6 +19,7 @@
 #define CUFE_IPFE_H
 
 #include <stdint.h>
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -183,6 +184,69 @@ int cufe_ctx_get_row(const cufe_ctx *ctx, int row, uint32_t *cl, uint32_t *c0);
 /* Stored rows [row0, row0+n) in compact wire form: c_L [n][3][4096] and
  * coefficient 0 of c_0..c_{l-1} as [n][l][3] (what a serving request carries). */
 int cufe_ctx_export_rows(const cufe_ctx *ctx, int row0, int n, uint32_t *cl, uint32_t *c0);
+
+/* Batched encryption of messages of uneven lengths (rlwe_sife_encrypt,     */
+/* src/rlwe_sife.c:112-181, once per message, as rlwe_sife_encrypt_gpu's    */
+/* repeat loop, src/rlwe_sife_gpu.cu:102, but with a length per message).  */
+/* Message b of length l_b (1 <= l_b <= key length) uses nonce nonce0 + b;  */
+/* its ciphertext [l_b+1][3][4096] (index l_b = c_L) equals c_0..c_{l_b-1}  */
+/* and c_L of the message zero-padded to the full key length.  Ciphertexts  */
+/* stay in HBM, packed in message order.  The context must outlive the     */
+/* batch.                                                                  */
+typedef struct cufe_encbatch cufe_encbatch;
+int cufe_encbatch_create(const cufe_ctx *ctx, const int *lens, int n, cufe_encbatch **out);
+/* x: packed messages (sum of lens words).  use_graph: replay the batch as one
+ * captured hipGraph (captured on first use).  ms: device time of the run. */
+int cufe_encbatch_run(cufe_encbatch *eb, const uint32_t *x, const unsigned char seed2[32],
+                      const unsigned char seed3[32], int nonce0, int use_graph, float *ms);
+/* Ciphertexts of messages [b0, b0+nb) to host memory, packed. */
+int cufe_encbatch_export(const cufe_encbatch *eb, int b0, int nb, uint32_t *c);
+/* Device checksum of all ciphertext words: sum of (2i+1) w_i mod 2^64.  info
+ * (optional): messages, words, sampler groups, groups redone. */
+int cufe_encbatch_checksum(cufe_encbatch *eb, uint64_t *sum, long long *info);
+void cufe_encbatch_destroy(cufe_encbatch *eb);
+
+/* ===================================================================== */
+/* 5. Serving: continuous batching over resident keys                      */
+/*    Replaces the one-row-per-call loop of svm_predict_gpu (main.c:270-390, */
+/*    which ships msk and a full ciphertext per row and regenerates every   */
+/*    functional key).  Requests carry rows in compact form (c_L and         */
+/*    coefficient 0 of c_0..c_{l-1}, as cufe_ctx_export_rows gives them); a */
+/*    worker thread packs queued requests of any loaded models into batches */
+/*    of up to the slot capacity, which the HBM budget fixes at start.      */
+/*    Results equal cufe_ctx_predict's for the same rows, alone or mixed.   */
+/* ===================================================================== */
+
+typedef struct cufe_server cufe_server;
+
+/* ctx supplies the master keys and must outlive the server.  hbm_budget:
+ * device bytes for keys and both batch slots (0 = 16 GiB); max_batch_rows:
+ * largest batch (0 = 4096); batching 0 runs one request per batch. */
+int cufe_server_create(cufe_ctx *ctx, size_t hbm_budget, int max_batch_rows, int batching, cufe_server **out);
+/* Generates and keeps the functional keys of one model (before start). */
+int cufe_server_add_model(cufe_server *srv, const cufe_svm_model *model, int *model_id);
+/* Sizes and allocates the batch slots and starts the worker thread. */
+int cufe_server_start(cufe_server *srv);
+/* Queues n rows of l slots (model's largest feature < l <= key length):
+ * cl [n][3][4096], c0 [n][l][3].  labels [n] and kvalues [n][model l]
+ * (optional) receive the results; every buffer stays the caller's and must
+ * live until the request completes. */
+int cufe_server_submit(cufe_server *srv, int model, int n, int l, const uint32_t *cl, const uint32_t *c0,
+                       double *labels, uint32_t *kvalues, uint64_t *ticket);
+/* 1 done, 0 pending after timeout_ms (< 0 waits without limit), -1 failed.
+ * A completed ticket is released.  latency_ms: submit to completion. */
+int cufe_server_wait(cufe_server *srv, uint64_t ticket, double timeout_ms, double *latency_ms);
+/* Open-loop replay of nreq requests at the given arrival offsets (seconds);
+ * latency_ms[i] is measured from the scheduled arrival.  Returns the number
+ * of failed requests, or -1. */
+int cufe_server_replay(cufe_server *srv, int nreq, const double *arrival_s, const int *model, const int *nrows,
+                       const int *l, const uint32_t *const *cl, const uint32_t *const *c0, double *const *labels,
+                       double *latency_ms, double *makespan_s);
+/* requests, rows, batches, failed, largest batch rows, slot capacity rows,
+ * model key bytes, bytes per slot. */
+int cufe_server_stats(cufe_server *srv, long long *out8);
+/* Completes queued requests, stops the worker, frees device memory. */
+void cufe_server_destroy(cufe_server *srv);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,7 @@ def lib() -> C.CDLL:
         "cufe_time_ntt": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "cufe_time_gauss": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "cufe_microbench": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
+        "cufe_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
         "rlwe_sife_setup_gpu": (None, [_u32p, _u32p, _u8p, _u8p, C.POINTER(C.c_float)]),
         "rlwe_sife_encrypt_gpu": (None, [_u32p, _u32p, _u32p, _u8p, _u8p, C.c_int, C.POINTER(C.c_float)]),
         "rlwe_sife_keygen_gpu": (None, [_u32p, _u32p, _u32p, C.c_int, C.POINTER(C.c_float)]),
@@ -205,6 +206,14 @@ def microbench(kind: int) -> float:
     return r.value
 
 
+def autotune_report() -> str:
+    """Block-shape choices of the int8 contraction so far."""
+    n = lib().cufe_autotune_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().cufe_autotune_report(buf, n + 1)
+    return buf.value.decode()
+
+
 def time_gauss(nstreams: int, which: int, iters: int) -> float:
     ms = C.c_float()
     _check(lib().cufe_time_gauss(nstreams, which, iters, C.byref(ms)), "time_gauss")
@@ -258,6 +267,7 @@ def _ctx_sigs(lb):
         "cufe_ctx_get_model_keys": (C.c_int, [p, _u32p]),
         "cufe_ctx_get_row": (C.c_int, [p, C.c_int, _u32p, _u32p]),
         "cufe_ctx_export_rows": (C.c_int, [p, C.c_int, C.c_int, _u32p, _u32p]),
+        "cufe_ctx_contract_digits": (C.c_int, [p]),
         "cufe_svm_predict_gpu": (C.c_double, [C.POINTER(_SvmModelC), _u32p, _u32p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
@@ -341,6 +351,10 @@ class Context:
         mx = (C.c_uint * 3)()
         _check(lib().cufe_ctx_model_info(self._h, kb, mx), "ctx_model_info")
         return list(kb), list(mx)
+
+    def contract_digits(self) -> int:
+        """Key digits of the int8 contraction (2 or 3), 0 for the fp64 kernels."""
+        return lib().cufe_ctx_contract_digits(self._h)
 
     def model_keys(self):
         sk = np.empty((self.model.l, NMODULI, N), np.uint32)

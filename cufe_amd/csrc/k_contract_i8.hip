@@ -1,0 +1,299 @@
+// k_contract_i8.hip -- the coefficient-0 contraction on int8 matrix cores,
+// exact.
+//
+// D[r][v] = sum_k A[r][k] B[v][k] mod q with balanced operands: row values
+// |a| <= (q-1)/2 < 2^30 and key values |b| <= 32512 (two key digits) or
+// |b| <= 8322943 (three).  Each value is split into signed base-256 digits,
+// a = sum_d a_d 256^d (4 digits) and b = sum_e b_e 256^e (NB digits), every
+// a_d, b_e in [-128, 127].  Then
+//     sum_k a b = sum_e 256^e T_e,   T_e = sum_d 256^d P_de,
+//     P_de = sum_k a_d[k] b_e[k],
+// and each P_de is one int8 GEMM with exact int32 accumulation:
+// |P_de| <= K * 2^14 < 2^31 for K < 2^17.  The epilogue forms each T_e in 64
+// bits (|T_e| < 2^56), reduces it mod q, and combines the residues with the
+// weights 256^e < q, so D is the canonical residue of the exact integer dot
+// product -- the value the fp64 kernels produce.
+//
+// Digit stores are laid out as MFMA fragments so every load is contiguous:
+//   rows, per limb: [row block of 32][k step of 32][digit d][lane][16 B]
+//   keys, per limb: [key block of 32][k step of 32][digit e < NB][lane][16 B]
+// lane l holds row / key (l mod 32) and k = 16 (l / 32) + j, j < 16 -- the
+// operand layout of v_mfma_i32_32x32x32_i8, whose result lane l, register i
+// is D[8 (i / 4) + 4 (l / 32) + i mod 4][l mod 32].
+#include "kernels_predict.h"
+
+namespace ipfe {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int DG_A = 4;  // digits per row value (key values: NB = 2 or 3)
+constexpr int FRAG_V4 = 64;        // one fragment: 64 lanes x 16 bytes
+
+// ---------------------------------------------------------------------------
+// Block tile: WR row blocks x WK key blocks of 32, one wave per 32 x 32 output
+// block with DG_A x DG_B int32 accumulators.  Per k step the block stages the
+// row digits (WR x 4 KB) and key digits (WK x NB KB) in LDS, double buffered,
+// with 16-byte loads spread over the block's threads.  Tiles are ordered so
+// that the blocks one XCD receives take consecutive tiles (the key tiles of one
+// row tile), sharing the row digits in that XCD's L2.
+// ---------------------------------------------------------------------------
+template <int WR, int WK, int NB>
+constexpr int i8_min_waves()
+{
+    // three waves per SIMD where the accumulators and prefetch registers fit
+    // in 168 VGPRs, else two
+    return (NB == 2 && WR * WK <= 4 && (WR * 256 + WK * NB * 64 + 64 * WR * WK - 1) / (64 * WR * WK) <= 3) ? 3 : 2;
+}
+
+template <int WR, int WK, int NB>
+__global__ __launch_bounds__(64 * WR * WK, (i8_min_waves<WR, WK, NB>())) void k_contract_i8(const v4i *__restrict__ A, const v4i *__restrict__ B,
+                                                               int R, int S, int ksteps, long long a_stride,
+                                                               long long b_stride, uint32_t q,
+                                                               uint32_t *__restrict__ D, int ntk, int ntiles)
+{
+    constexpr int NT = 64 * WR * WK;
+    constexpr int SA = WR * DG_A * FRAG_V4, SB = WK * NB * FRAG_V4;  // v4i per k step
+    constexpr int NSL = (SA + SB + NT - 1) / NT;                       // loads per thread
+    __shared__ v4i lds[2][SA + SB];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int per = gridDim.x / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (t >= ntiles) return;
+    const int tk = t % ntk, tr = t / ntk;
+    const int wr = w / WK, wk = w % WK;
+    // load slot u of this thread: slot tid + u NT; slots [0, SA) are row digits
+    // (row block slot / (DG_A 64)), the rest key digits
+    const v4i *src[NSL];
+    long long step[NSL];
+#pragma unroll
+    for (int u = 0; u < NSL; ++u) {
+        const int sl = tid + u * NT;
+        if (sl < SA) {
+            src[u] = A + (size_t)(tr * WR + sl / (DG_A * FRAG_V4)) * a_stride + sl % (DG_A * FRAG_V4);
+            step[u] = DG_A * FRAG_V4;
+        } else {
+            const int sb = sl < SA + SB ? sl - SA : 0;
+            src[u] = B + (size_t)(tk * WK + sb / (NB * FRAG_V4)) * b_stride + sb % (NB * FRAG_V4);
+            step[u] = NB * FRAG_V4;
+        }
+    }
+    v16i acc[DG_A][NB];
+#pragma unroll
+    for (int d = 0; d < DG_A; ++d)
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+    v4i pre[NSL];
+#pragma unroll
+    for (int u = 0; u < NSL; ++u) {
+        pre[u] = src[u][0];
+        if (tid + u * NT < SA + SB) lds[0][tid + u * NT] = pre[u];
+    }
+    __syncthreads();
+    for (int ks = 0; ks < ksteps; ++ks) {
+        const int cur = ks & 1;
+        const bool more = ks + 1 < ksteps;
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < NSL; ++u) pre[u] = src[u][(size_t)(ks + 1) * step[u]];
+        }
+        v4i fa[DG_A], fb[NB];
+#pragma unroll
+        for (int d = 0; d < DG_A; ++d) fa[d] = lds[cur][wr * (DG_A * FRAG_V4) + d * FRAG_V4 + lane];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) fb[e] = lds[cur][SA + wk * (NB * FRAG_V4) + e * FRAG_V4 + lane];
+#pragma unroll
+        for (int d = 0; d < DG_A; ++d)
+#pragma unroll
+            for (int e = 0; e < NB; ++e)
+                acc[d][e] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d], fb[e], acc[d][e], 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < NSL; ++u)
+                if (tid + u * NT < SA + SB) lds[cur ^ 1][tid + u * NT] = pre[u];
+        }
+        __syncthreads();
+    }
+    const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
+    if (key >= S) return;
+    const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
+    const double qinv = 1.0 / (double)q;
+    const long long qq = q;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = row0 + (i >> 2) * 8 + (i & 3);
+        if (row >= R) continue;
+        long long tot = 0;
+#pragma unroll
+        for (int e = NB - 1; e >= 0; --e) {
+            long long T = 0;
+#pragma unroll
+            for (int d = 0; d < DG_A; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
+            // |T| < 2^56 and then |tot| < 2^40: the double quotient is off by
+            // at most one
+            long long m = T - (long long)floor((double)T * qinv) * qq;
+            m += (m < 0) ? qq : 0;
+            m -= (m >= qq) ? qq : 0;
+            tot = tot * 256 + m;
+            tot -= (long long)floor((double)tot * qinv) * qq;
+            tot += (tot < 0) ? qq : 0;
+            tot -= (tot >= qq) ? qq : 0;
+        }
+        D[(size_t)row * S + key] = (uint32_t)tot;
+    }
+}
+
+__device__ __forceinline__ int digit_split(int &x)
+{
+    const int d = ((x + 128) & 255) - 128;
+    x = (x - d) / 256;
+    return d;
+}
+
+// Balanced int32 rows [3][cap][ldr] -> row digit store, rows [r0, r0 + R).
+// One thread per (limb, row, 4 consecutive k): one 32-bit word per digit plane.
+__global__ void k_rows_digits(const int32_t *__restrict__ a32, int cap, int ldr, int r0, int R, int ksteps,
+                              size_t limb_bytes, int8_t *__restrict__ dig)
+{
+    const int quads = ksteps * 8;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)3 * R * quads) return;
+    const int s = (int)(i / ((size_t)R * quads));
+    const size_t rem = i % ((size_t)R * quads);
+    const int row = r0 + (int)(rem / quads), k = 4 * (int)(rem % quads);
+    int v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = (k + u < ldr) ? a32[((size_t)s * cap + row) * ldr + k + u] : 0;
+    const int ks = k >> 5, kk = k & 31;
+    const int ln = (kk >> 4) * 32 + (row & 31), j = kk & 15;
+    int8_t *base = dig + s * limb_bytes + ((size_t)(row >> 5) * ksteps + ks) * DG_A * 1024 + ln * 16 + j;
+#pragma unroll
+    for (int d = 0; d < DG_A; ++d) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) word |= (uint32_t)(uint8_t)(int8_t)digit_split(v[u]) << (8 * u);
+        *reinterpret_cast<uint32_t *>(base + (size_t)d * 1024) = word;
+    }
+}
+
+// Balanced fp64 keys [3][S][K] -> key digit store [3][spad/32][ksteps][nb][..],
+// keys >= S and k >= K zero.  Values must fit nb signed digits.
+__global__ void k_keys_digits(const double *__restrict__ kd, int S, int K, int spad, int ksteps, int nb,
+                              int8_t *__restrict__ dig)
+{
+    const int quads = ksteps * 8;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)3 * spad * quads) return;
+    const int s = (int)(i / ((size_t)spad * quads));
+    const size_t rem = i % ((size_t)spad * quads);
+    const int key = (int)(rem / quads), k = 4 * (int)(rem % quads);
+    int v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        v[u] = (key < S && k + u < K) ? (int)kd[((size_t)s * S + key) * K + k + u] : 0;
+    const int ks = k >> 5, kk = k & 31;
+    const int ln = (kk >> 4) * 32 + (key & 31), j = kk & 15;
+    const size_t limb_bytes = (size_t)spad / 32 * ksteps * nb * 1024;
+    int8_t *base = dig + s * limb_bytes + ((size_t)(key >> 5) * ksteps + ks) * nb * 1024 + ln * 16 + j;
+    for (int e = 0; e < nb; ++e) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) word |= (uint32_t)(uint8_t)(int8_t)digit_split(v[u]) << (8 * u);
+        *reinterpret_cast<uint32_t *>(base + (size_t)e * 1024) = word;
+    }
+}
+
+// 128 rows past the capacity: a tile of up to 4 row blocks may start at any
+// 32-row boundary below the capacity
+size_t rows_digit_limb_bytes(int cap, int ksteps) { return (size_t)(cap + 128) / 32 * ksteps * DG_A * 1024; }
+size_t keys_digit_limb_bytes(int spad, int ksteps, int nb) { return (size_t)spad / 32 * ksteps * nb * 1024; }
+
+int launch_rows_digits(const int32_t *a32, int cap, int ldr, int r0, int R, int ksteps, int8_t *dig, hipStream_t st)
+{
+    const size_t n = (size_t)3 * R * ksteps * 8;
+    if (n == 0) return 0;
+    if (ksteps * 32 < ldr || r0 < 0 || r0 + R > cap) {
+        ipfe_set_error("row digits: %d k steps for %d columns, rows [%d, %d) of %d", ksteps, ldr, r0, r0 + R, cap);
+        return -1;
+    }
+    hipLaunchKernelGGL(k_rows_digits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a32, cap, ldr, r0, R,
+                       ksteps, rows_digit_limb_bytes(cap, ksteps), dig);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int nb, int8_t *dig, hipStream_t st)
+{
+    const size_t n = (size_t)3 * spad * ksteps * 8;
+    if (n == 0) return 0;
+    if (spad % 128 || spad < S || ksteps * 32 < K || nb < 2 || nb > 3) {
+        ipfe_set_error("key digits: %d keys padded to %d, %d k steps for K=%d", S, spad, ksteps, K);
+        return -1;
+    }
+    hipLaunchKernelGGL(k_keys_digits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, kd, S, K, spad, ksteps, nb,
+                       dig);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// Block shapes (row blocks x key blocks) the launcher can run.
+static const int kShapes[][2] = {{2, 4}, {2, 2}, {4, 2}, {1, 4}, {4, 1}};
+constexpr int kNumShapes = 5;
+
+int i8_shape_count() { return kNumShapes; }
+void i8_shape(int v, int *wr, int *wk)
+{
+    *wr = kShapes[v][0];
+    *wk = kShapes[v][1];
+}
+
+// A: row digit store of one limb at row r0 (r0 a multiple of 32; the store
+// holds 128 rows past its capacity), ksteps_a its k steps; B: key digit store
+// of one limb with nb digits, ksteps k steps (<= ksteps_a), keys padded to a
+// multiple of 128.  variant: index into the block shapes.
+int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
+                       uint32_t *D, hipStream_t st, int variant)
+{
+    if (R <= 0 || S <= 0) return 0;
+    if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 17) || variant < 0 || variant >= kNumShapes ||
+        nb < 2 || nb > 3) {
+        ipfe_set_error("int8 contraction: %d k steps (row store %d), variant %d, %d key digits", ksteps, ksteps_a,
+                       variant, nb);
+        return -1;
+    }
+    const int WR = kShapes[variant][0], WK = kShapes[variant][1];
+    // row tiles never reach past the 128 padding rows; key tiles stay within
+    // the 128-key padding
+    const int ntk = (S + 32 * WK - 1) / (32 * WK), ntr = (R + 32 * WR - 1) / (32 * WR);
+    const int ntiles = ntk * ntr;
+    const dim3 grid(((ntiles + 7) / 8) * 8), block(64 * WR * WK);
+    const v4i *a = reinterpret_cast<const v4i *>(A);
+    const v4i *b = reinterpret_cast<const v4i *>(B);
+    const long long as = (long long)ksteps_a * DG_A * FRAG_V4, bs = (long long)ksteps * nb * FRAG_V4;
+    const uint32_t q = qmod(s);
+#define CI8(wr, wk, n) \
+    hipLaunchKernelGGL((k_contract_i8<wr, wk, n>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
+    if (nb == 2) {
+        switch (variant) {
+        case 0: CI8(2, 4, 2); break;
+        case 1: CI8(2, 2, 2); break;
+        case 2: CI8(4, 2, 2); break;
+        case 3: CI8(1, 4, 2); break;
+        default: CI8(4, 1, 2); break;
+        }
+    } else {
+        switch (variant) {
+        case 0: CI8(2, 4, 3); break;
+        case 1: CI8(2, 2, 3); break;
+        case 2: CI8(4, 2, 3); break;
+        case 3: CI8(1, 4, 3); break;
+        default: CI8(4, 1, 3); break;
+        }
+    }
+#undef CI8
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace ipfe

@@ -29,6 +29,20 @@ int launch_contract(const int32_t *A, const double *B, int R, int S, int K, int 
 // of 16, [k][16] per group, cap a multiple of 64), lda = its column count.
 int launch_contract_sgpr(const double *A, const double *B, int R, int S, int K, int lda, int ldb, int s, int kb,
                          uint32_t *D, hipStream_t st);
+// int8-digit contraction (k_contract_i8.hip): digit stores and the exact
+// MFMA contraction of one limb.
+size_t rows_digit_limb_bytes(int cap, int ksteps);
+size_t keys_digit_limb_bytes(int spad, int ksteps, int nb);
+int launch_rows_digits(const int32_t *a32, int cap, int ldr, int r0, int R, int ksteps, int8_t *dig, hipStream_t st);
+int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int nb, int8_t *dig, hipStream_t st);
+int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
+                       uint32_t *D, hipStream_t st, int variant = 0);
+int i8_shape_count();
+void i8_shape(int variant, int *wr, int *wk);
+// largest key coefficients the two- and three-digit key splits hold: the top
+// digit (|b| + 128 (1 + 256 + ...)) / 256^(nb-1) must stay within [-128, 127]
+constexpr unsigned int I8_KEY_MAX2 = 32512;
+constexpr unsigned int I8_KEY_MAX3 = 8322943;
 int launch_rows_blocked(const int32_t *a32, int cap, int cap64, int ldr, int r0, int R, double *a64, hipStream_t st);
 int launch_crt_kvalue(const uint32_t *D, size_t n, uint32_t *kval, hipStream_t st);
 int launch_kvalue(const uint32_t *D, const uint32_t *c0, int l, const SparseY &y, int R, int S, uint32_t *kval,

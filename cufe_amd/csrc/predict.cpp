@@ -13,6 +13,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
 #include <vector>
 
 #include "resident.h"
@@ -25,10 +29,80 @@ constexpr size_t kPoly = (size_t)3 * IPFE_N;
 
 namespace ipfe {
 
+// ---------------------------------------------------------------------------
+// Block-shape autotuner of the int8 contraction.  The first launch of a shape
+// (keys, log2 of the rows, k steps) times every block shape on the real
+// operands -- all produce the same exact residues -- and keeps the fastest;
+// later launches of that shape use it.  CUFE_I8_VARIANT=v forces shape v.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tune;
+std::string g_tune_log;
+
+int ilog2(int x)
+{
+    int r = 0;
+    while (x > 1) {
+        x >>= 1;
+        ++r;
+    }
+    return r;
+}
+}  // namespace
+
+int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
+                      uint32_t *D, hipStream_t st)
+{
+    const char *env = getenv("CUFE_I8_VARIANT");
+    if (env && *env) return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, atoi(env));
+    const auto key = std::make_tuple(S, ilog2(R), ksteps * 4 + nb);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, it->second);
+    }
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    // one untimed launch loads the code objects
+    if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, 0)) return -1;
+    int best = 0;
+    float best_ms = 1e30f;
+    char line[256];
+    int off = snprintf(line, sizeof line, "S=%d R=%d ksteps=%d key digits=%d:", S, R, ksteps, nb);
+    for (int v = 0; v < i8_shape_count(); ++v) {
+        HIP_OK(hipEventRecord(e0, st));
+        if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v)) return -1;
+        HIP_OK(hipEventRecord(e1, st));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        int wr, wk;
+        i8_shape(v, &wr, &wk);
+        off += snprintf(line + off, sizeof line - off, " %dx%d %.3f ms", wr, wk, ms);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int wr, wk;
+    i8_shape(best, &wr, &wk);
+    snprintf(line + off, sizeof line - off, " -> %dx%d\n", wr, wk);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+    g_tune_log += line;
+    return 0;  // D holds the last variant's (identical) result
+}
+
 int contract_mode()
 {
     const char *m = getenv("CUFE_CONTRACT");
-    return (m && strcmp(m, "lds") == 0) ? 0 : 1;
+    if (m && strcmp(m, "lds") == 0) return 0;
+    if (m && strcmp(m, "sgpr") == 0) return 1;
+    return 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -116,6 +190,16 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
         double k = bmax > 0 ? floor(room / (16.0 * floor(q / 2) * bmax)) : 256.0;
         kb[s] = (int)std::min(256.0, k);
     }
+    const unsigned int mx3 = std::max(maxabs[0], std::max(maxabs[1], maxabs[2]));
+    nb = mx3 <= I8_KEY_MAX2 ? 2 : (mx3 <= I8_KEY_MAX3 ? 3 : 0);
+    i8 = nb > 0;
+    kdig.reset();
+    if (i8) {
+        spad = (nS + 127) / 128 * 128;
+        ksteps = (kk + 31) / 32;
+        if (kdig.alloc(3 * keys_digit_limb_bytes(spad, ksteps, nb))) return -1;
+        if (launch_keys_digits(kd.get(), nS, kk, spad, ksteps, nb, kdig.get(), st) || sync_stream(st)) return -1;
+    }
     S = nS;
     nr_class = m->nr_class;
     maxidx = mx;
@@ -125,7 +209,7 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
     return 0;
 }
 
-size_t Model::bytes() const { return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8); }
+size_t Model::bytes() const { return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8) + kdig.n; }
 
 // ---------------------------------------------------------------------------
 // RowStore
@@ -133,7 +217,8 @@ size_t Model::bytes() const { return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey *
 size_t RowStore::bytes_per_row(int l)
 {
     const size_t ldr = IPFE_N + ((l + 15) / 16) * 16;
-    return 3 * ldr * (4 + 8) + (size_t)l * 12;
+    const size_t kd = IPFE_N + ((l + 31) / 32) * 32;
+    return 3 * ldr * (4 + 8) + 3 * kd * 4 + (size_t)l * 12;
 }
 
 int RowStore::reserve(int cap_rows, int len)
@@ -144,13 +229,17 @@ int RowStore::reserve(int cap_rows, int len)
     }
     const int ld = IPFE_N + ((len + 15) / 16) * 16;
     const int cp = ((cap_rows + 63) / 64) * 64;  // whole 64-row tiles of the blocked store
-    if (A.alloc((size_t)3 * cp * ld) || c0.alloc((size_t)cp * len * 3) || A64.alloc((size_t)3 * (cp + 64) * ld))
+    const int ks = (IPFE_N + ((len + 31) / 32) * 32) / 32;
+    if (A.alloc((size_t)3 * cp * ld) || c0.alloc((size_t)cp * len * 3) || A64.alloc((size_t)3 * (cp + 64) * ld) ||
+        Adig.alloc(3 * rows_digit_limb_bytes(cp, ks)))
         return -1;
     HIP_OK(hipMemset(A.get(), 0, (size_t)3 * cp * ld * 4));
     HIP_OK(hipMemset(A64.get(), 0, (size_t)3 * (cp + 64) * ld * 8));
+    HIP_OK(hipMemset(Adig.get(), 0, Adig.n));
     cap = cp;
     l = len;
     ldr = ld;
+    ksteps = ks;
     return 0;
 }
 
@@ -167,6 +256,7 @@ int RowStore::put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev,
     HIP_OK(hipMemcpy2DAsync(c0.get() + (size_t)r0 * l * 3, (size_t)l * 12, c0_dev, (size_t)src_l * 12,
                             (size_t)src_l * 12, n, hipMemcpyDeviceToDevice, st));
     if (launch_rows_blocked(A.get(), cap, cap + 64, ldr, r0, n, A64.get(), st)) return -1;
+    if (launch_rows_digits(A.get(), cap, ldr, r0, n, ksteps, Adig.get(), st)) return -1;
     return 0;
 }
 
@@ -208,9 +298,14 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
             const double *Bs = m.kd.get() + (size_t)s * S * m.kkey;
             uint32_t *Ds = sc.D.get() + (size_t)s * R * S;
             int rc;
-            if (m.kb[s] < 1)
+            if (mode == 2 && m.i8 && r0 % 32 == 0)
+                rc = contract_i8_tuned(rs.Adig.get() + s * rows_digit_limb_bytes(rs.cap, rs.ksteps) +
+                                           (size_t)(r0 / 32) * rs.ksteps * 4 * 1024,
+                                       rs.ksteps, m.kdig.get() + s * keys_digit_limb_bytes(m.spad, m.ksteps, m.nb),
+                                       m.ksteps, m.nb, R, S, s, Ds, st);
+            else if (m.kb[s] < 1)
                 rc = launch_contract_slow(As, Bs, R, S, m.kkey, rs.ldr, m.kkey, s, dev->barrett, Ds, st);
-            else if (mode == 1 && r0 % 16 == 0)
+            else if (mode >= 1 && r0 % 16 == 0)
                 // kb counts 16-product steps; this kernel's steps are 8 products
                 rc = launch_contract_sgpr(rs.A64.get() + ((size_t)s * (rs.cap + 64) + r0) * rs.ldr, Bs, R, S, m.kkey,
                                           rs.ldr, m.kkey, s, 2 * m.kb[s], Ds, st);
@@ -534,6 +629,17 @@ extern "C" int cufe_ctx_model_info(const cufe_ctx *c, int *kb, unsigned int *max
     return 0;
 }
 
+// Key digits of the int8 contraction for the loaded model (2 or 3), or 0 when
+// its key coefficients are too large and the fp64 kernels run.
+extern "C" int cufe_ctx_contract_digits(const cufe_ctx *c)
+{
+    if (!c || c->model.S < 1) {
+        ipfe_set_error("cufe_ctx_contract_digits: no model");
+        return -1;
+    }
+    return c->model.nb;
+}
+
 // The resident functional keys [S][3][N] (tests).
 extern "C" int cufe_ctx_get_model_keys(const cufe_ctx *c, uint32_t *sk)
 {
@@ -571,4 +677,13 @@ extern "C" int cufe_ctx_export_rows(const cufe_ctx *c, int row0, int n, uint32_t
 extern "C" int cufe_ctx_get_row(const cufe_ctx *c, int row, uint32_t *cl, uint32_t *c0)
 {
     return cufe_ctx_export_rows(c, row, 1, cl, c0);
+}
+
+// Block-shape choices of the int8 contraction made so far, one line per shape
+// with every candidate's time.  Returns the length of the full report.
+extern "C" int cufe_autotune_report(char *buf, int len)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", g_tune_log.c_str());
+    return (int)g_tune_log.size();
 }

@@ -27,6 +27,11 @@ struct Model {
     DevBuf<double> kd;    // [3][S][kkey]
     DevBuf<double> coef, rho;
     SvmModelDev mdev{};
+    // int8-digit key store [3][spad/32][ksteps][nb][64][16] when every key
+    // coefficient fits nb = 2 or 3 signed digits (nb = 0: fp64 kernels only)
+    bool i8 = false;
+    int nb = 0, spad = 0, ksteps = 0;
+    DevBuf<int8_t> kdig;
 
     int build(const uint32_t *msk_dev, int l_key, const cufe_svm_model *m, hipStream_t st);
     size_t bytes() const;
@@ -35,11 +40,14 @@ struct Model {
 // Compact encrypted rows of l slots: c_L and coefficient 0 of c_0..c_{l-1},
 // kept as [3][cap][ldr] int32 (balanced; ldr = N + l rounded up to 16), the
 // same rows blocked for the scalar-operand contraction ([3][(cap+64)/16][ldr][16]
-// fp64), and [cap][l][3] canonical coefficient-0 residues.
+// fp64), as int8 digit fragments for the matrix-core contraction
+// ([3][(cap+128)/32][ksteps][4][64][16], ksteps = (N + l rounded up to 32) / 32),
+// and [cap][l][3] canonical coefficient-0 residues.
 struct RowStore {
-    int l = 0, cap = 0, ldr = 0;
+    int l = 0, cap = 0, ldr = 0, ksteps = 0;
     DevBuf<int32_t> A;
     DevBuf<double> A64;
+    DevBuf<int8_t> Adig;
     DevBuf<uint32_t> c0;
 
     int reserve(int cap, int l);
@@ -72,8 +80,10 @@ int predict_rows(const Model &m, const RowStore &rs, int r0, int R, Scratch &sc,
 int encrypt_rows_compact(const uint32_t *mpk_dev, int l_key, RowStore &rs, int r0, const uint32_t *x, int n,
                          const unsigned char seed2[32], const unsigned char seed3[32], int nonce0, hipStream_t st);
 
-// Contraction kernel choice: 1 = scalar-operand (default), 0 = LDS tiles
-// (CUFE_CONTRACT=lds).
+// Contraction kernel choice: 2 = int8 matrix cores (default), 1 = fp64
+// scalar-operand (CUFE_CONTRACT=sgpr), 0 = fp64 LDS tiles (CUFE_CONTRACT=lds).
+// The int8 kernel needs key coefficients within two digits and rows starting
+// on a 32-row boundary; otherwise the fp64 kernels run.
 int contract_mode();
 
 }  // namespace ipfe

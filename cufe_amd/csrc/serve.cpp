@@ -107,7 +107,7 @@ int ctx_key_len(const cufe_ctx *c);
 
 namespace {
 
-inline int round16(int x) { return (x + 15) & ~15; }
+inline int round32(int x) { return (x + 31) & ~31; }
 
 // Stages the batch in sl (host copies, H2D, row-store puts) and launches its
 // prediction; returns -1 with the error set if a call fails.
@@ -115,7 +115,7 @@ int launch_batch(cufe_server *s, Slot &sl)
 {
     const int mode = contract_mode();
     // layout: requests grouped by model in order of first appearance, each
-    // group starting on a 16-row boundary (the blocked store's row groups)
+    // group starting on a 32-row boundary (the digit store's row blocks)
     std::vector<int> order;
     std::vector<const Model *> gm;
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
@@ -137,7 +137,7 @@ int launch_batch(cufe_server *s, Slot &sl)
             gr.want_kv |= sl.reqs[i]->kvalues != nullptr;
         }
         kv += (size_t)gr.R * gm[g]->S;
-        row = round16(row + gr.R);
+        row = round32(row + gr.R);
         sl.groups.push_back(gr);
     }
     if (row > s->cap) {
@@ -254,8 +254,8 @@ void worker_main(cufe_server *s)
                     auto it = std::find_if(per_model.begin(), per_model.end(),
                                            [&](const std::pair<const Model *, int> &p) { return p.first == r->m; });
                     int need = 0;
-                    for (auto &p : per_model) need += round16(p.second + (p.first == r->m ? r->n : 0));
-                    if (it == per_model.end()) need += round16(r->n);
+                    for (auto &p : per_model) need += round32(p.second + (p.first == r->m ? r->n : 0));
+                    if (it == per_model.end()) need += round32(r->n);
                     if (need > s->cap) break;
                     if (it == per_model.end())
                         per_model.push_back({r->m, r->n});
@@ -308,7 +308,7 @@ int submit_req(cufe_server *s, int model, int n, int l, const uint32_t *cl, cons
         return -1;
     }
     const Model *m = s->models[model].get();
-    if (n < 1 || round16(n) > s->cap) {
+    if (n < 1 || round32(n) > s->cap) {
         ipfe_set_error("cufe_server_submit: %d rows (slot capacity %d)", n, s->cap);
         return -1;
     }

@@ -180,6 +180,15 @@ int cufe_ctx_model_info(const cufe_ctx *ctx, int *kb, unsigned int *maxabs);
 /* Functional keys [model l][3][4096] and one stored row (c_L [3][4096] and
  * coefficient 0 of c_0..c_{l-1} as [l][3]), for inspection. */
 int cufe_ctx_get_model_keys(const cufe_ctx *ctx, uint32_t *sk);
+/* The contraction runs on int8 matrix cores when every functional-key
+ * coefficient fits two signed base-256 digits (|b| <= 32512), else on fp64;
+ * CUFE_CONTRACT=sgpr|lds forces an fp64 kernel.  The int8 kernel's block shape
+ * is tuned on the first launch of each (keys, rows, k steps) shape; this
+ * report lists the candidates' times and choices.  Returns its length. */
+int cufe_autotune_report(char *buf, int len);
+/* Key digits of the loaded model's int8 contraction: 2 (|b| <= 32512), 3
+ * (|b| <= 8322943) or 0 (fp64 kernels). */
+int cufe_ctx_contract_digits(const cufe_ctx *ctx);
 int cufe_ctx_get_row(const cufe_ctx *ctx, int row, uint32_t *cl, uint32_t *c0);
 /* Stored rows [row0, row0+n) in compact wire form: c_L [n][3][4096] and
  * coefficient 0 of c_0..c_{l-1} as [n][l][3] (what a serving request carries). */

@@ -1,0 +1,106 @@
+"""The coefficient-0 contraction kernels agree bit for bit: the int8 matrix-core
+kernel in every block shape (and its autotuned choice), the fp64 scalar-operand
+kernel and the fp64 LDS kernel, on one model and one row store.  Keys too large
+for the two-digit split run on fp64 and still match the oracle."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 4096
+
+
+def h(t: bytes) -> bytes:
+    return hashlib.sha256(t).digest()
+
+
+def note(msg):
+    print(f"[gpu-contract] {msg}", flush=True)
+
+
+def model_with(gpu, rng, S, dim, hi):
+    sv = np.zeros((S, 785), np.uint32)
+    for v in range(S):
+        idx = rng.choice(np.arange(1, dim), size=8, replace=False)
+        sv[v, idx] = rng.integers(1, hi, 8)
+    coef = rng.integers(-16, 17, (1, S)).astype(np.float64)
+    return gpu.SvmModel(2, [S // 2, S - S // 2], [1, -1], coef, [0.5], sv)
+
+
+def predict_with(ctx, R, env):
+    old = {k: os.environ.get(k) for k in env}
+    try:
+        for k, v in env.items():
+            os.environ[k] = v
+        lab, kv, _ = ctx.predict(0, R, want_kvalues=True)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return lab, kv
+
+
+@pytest.mark.parametrize("hi,digits", [(2, 2), (17, 3)])
+def test_kernels_agree(gpu, hi, digits):
+    """Binary features give key coefficients within two digits; features up to
+    16 need three."""
+    rng = np.random.default_rng(31)
+    l, R, S = 124, 700, 300
+    ctx = gpu.Context()
+    ctx.setup(h(b"cA"), h(b"cS"), h(b"cE"), 785)
+    x = rng.integers(0, 5, (R, l)).astype(np.uint32)
+    ctx.rows_reserve(R, l)
+    ctx.encrypt_rows(x, h(b"c2"), h(b"c3"), nonce0=0)
+    model = model_with(gpu, rng, S, l, hi)
+    ctx.set_model(model)
+    kb, maxabs = ctx.model_info()
+    note(f"maxabs {maxabs} digits {ctx.contract_digits()}")
+    assert ctx.contract_digits() == digits
+    lab0, kv0 = predict_with(ctx, R, {})
+    ip = (x.astype(np.int64) @ model.sv[:, :l].astype(np.int64).T) % 50241
+    assert np.array_equal(kv0.astype(np.int64) % 50241, ip)
+    for env in [{"CUFE_I8_VARIANT": str(v)} for v in range(5)] + [{"CUFE_CONTRACT": "sgpr"},
+                                                                  {"CUFE_CONTRACT": "lds"}]:
+        note(f"kernel {env}")
+        lab, kv = predict_with(ctx, R, env)
+        assert np.array_equal(kv, kv0), env
+        assert np.array_equal(lab, lab0), env
+    note(gpu.autotune_report())
+    ctx.close()
+
+
+def test_large_keys_fall_back_to_fp64(gpu, po):
+    rng = np.random.default_rng(32)
+    l, R, S = 40, 5, 6
+    ctx = gpu.Context()
+    ctx.setup(h(b"LA"), h(b"LS"), h(b"LE"), 785)
+    mpk, msk = ctx.get_keys()
+    x = rng.integers(0, 5, (R, l)).astype(np.uint32)
+    ctx.rows_reserve(R, l)
+    ctx.encrypt_rows(x, h(b"L2"), h(b"L3"), nonce0=3)
+    model = model_with(gpu, rng, S, l, 60000)
+    ctx.set_model(model)
+    kb, maxabs = ctx.model_info()
+    note(f"maxabs {maxabs} kb {kb}")
+    assert ctx.contract_digits() == 0 and max(maxabs) > 8322943
+    _, kv, _ = ctx.predict(0, R, want_kvalues=True)
+    _, kv_lds = predict_with(ctx, R, {"CUFE_CONTRACT": "lds"})
+    assert np.array_equal(kv, kv_lds)
+    mpk_l = np.concatenate([mpk[:l], mpk[785:786]])
+    for r in (0, 4):
+        c_full = po.encrypt(x[r], mpk_l, h(b"L2"), h(b"L3"), 3 + r)
+        for v in (0, 5):
+            y_pad = np.zeros(l + 1, np.uint32)
+            y_pad[:l] = model.sv[v, :l]
+            sk = po.keygen(y_pad, msk[: l + 1])
+            c_pad = np.zeros((l + 2, 3, N), np.uint32)
+            c_pad[:l] = c_full[:l]
+            c_pad[l + 1] = c_full[l]
+            d = po.decrypt(c_pad, y_pad, sk)
+            assert int(kv[r, v]) == po.crt_round(int(d[0, 0]), int(d[1, 0]), int(d[2, 0]))
+    ctx.close()

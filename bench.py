@@ -121,6 +121,15 @@ def serving(cf, ctx, model, x, args, rng):
     ctx2.encrypt_rows(x2, h(b"serve-2"), h(b"serve-3"), nonce0=0)
     cl2, c02 = ctx2.export_rows(0, pool2)
 
+    def pinned(a):
+        # request rows live in pinned host memory, so the server DMAs them
+        # straight from the caller's buffers
+        p = cf.host_empty(a.shape, a.dtype)
+        p[...] = a
+        return p
+
+    cl1, c01, cl2, c02 = pinned(cl1), pinned(c01), pinned(cl2), pinned(c02)
+
     def svm(S, dim, width):
         svx = np.zeros((S, width), np.uint32)
         for v in range(S):
@@ -151,18 +160,22 @@ def serving(cf, ctx, model, x, args, rng):
     total_rows = int(sum(k for _, k in spans))
     res = {"arrival_rate_req_s": args.serve_rate, "requests": n, "rows": total_rows,
            "models": [{"svs": m.l, "row_slots": int(pools[p][2].shape[1])} for m, p in models]}
-    for mode, batching in (("batched", True), ("one_request_per_batch", False)):
+    runs = [("batched", True, 1), ("one_request_per_batch", False, 1)]
+    if args.serve_burst > 1:
+        runs += [(f"batched_x{args.serve_burst}", True, args.serve_burst),
+                 (f"one_request_per_batch_x{args.serve_burst}", False, args.serve_burst)]
+    for mode, batching, speed in runs:
         srv = cf.Server(ctx, max_batch_rows=4096, batching=batching)
         for m, _ in models:
             srv.add_model(m)
         srv.start()
-        log(f"serving replay ({mode}): {n} requests, {total_rows} rows")
-        labels, lat, makespan, failed = srv.replay(arrival, mids, rows)
+        log(f"serving replay ({mode}): {n} requests, {total_rows} rows at {args.serve_rate * speed:.0f} req/s")
+        labels, lat, makespan, failed = srv.replay(arrival / speed, mids, rows)
         st = srv.stats()
         srv.close()
         bad = sum(int(not np.array_equal(labels[i], refs[mids[i]][spans[i][0]:spans[i][0] + spans[i][1]]))
                   for i in range(n))
-        res[mode] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        res[mode] = {"arrival_rate_req_s": args.serve_rate * speed, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                      "p999_ms": float(np.percentile(lat, 99.9)), "mean_ms": float(np.mean(lat)),
                      "throughput_req_s": n / makespan, "throughput_rows_s": total_rows / makespan,
                      "makespan_s": makespan, "batches": st["batches"], "max_batch_rows": st["max_batch_rows"],
@@ -241,6 +254,10 @@ def main():
     ap.add_argument("--enc-messages", type=int, default=4096)
     ap.add_argument("--enc-plain", type=int, default=4)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy first (known bytes, to scale HBM counters)")
+    ap.add_argument("--serve-burst", type=float, default=10.0,
+                    help="also replay the same requests this many times faster")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in DESIGN.md)",
@@ -251,6 +268,8 @@ def main():
 
     info = cf.device_info()
     log(f"device {info}")
+    if args.copy_calibration:
+        log(f"copy calibration kernel: 1 GiB read + 1 GiB written per launch, {cf.microbench(2):.0f} GB/s")
     R, S = args.rows, args.svs
     rng = np.random.default_rng(20261202)
     x = sparse_binary_rows(rng, R)
@@ -317,7 +336,14 @@ def main():
         roof = {"bound": "valu-fp64", "kernel": "k_contract_sgpr", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "work_per_launch": f"{R} rows x {S} keys x {K} columns (1 fp64 FMA each) for one limb"}
     achieved = ops_per_launch / (avg_launch_ms * 1e-3) / 1e12
-    roof.update({"achieved": achieved, "frac": achieved / roof["peak"], "traffic": None,
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_contract_i8_traffic.json")
+    if i8 and os.path.exists(tfile):
+        with open(tfile) as f:
+            t = json.load(f)
+        if t.get("rows") == R and t.get("svs") == S:
+            traffic = t["hbm_bytes_per_launch"]
+    roof.update({"achieved": achieved, "frac": achieved / roof["peak"], "traffic": traffic,
                  "avg_launch_ms": avg_launch_ms,
                  "modular_macs_per_s": 3.0 * R * S * K * args.steps / (contract_ms * 1e-3)})
     out = {

@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
         "cufe_time_gauss": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "cufe_microbench": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
         "cufe_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
+        "cufe_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+        "cufe_host_free": (None, [C.c_void_p]),
         "rlwe_sife_setup_gpu": (None, [_u32p, _u32p, _u8p, _u8p, C.POINTER(C.c_float)]),
         "rlwe_sife_encrypt_gpu": (None, [_u32p, _u32p, _u32p, _u8p, _u8p, C.c_int, C.POINTER(C.c_float)]),
         "rlwe_sife_keygen_gpu": (None, [_u32p, _u32p, _u32p, C.c_int, C.POINTER(C.c_float)]),
@@ -204,6 +206,32 @@ def microbench(kind: int) -> float:
     r = C.c_double()
     _check(lib().cufe_microbench(kind, C.byref(r)), "microbench")
     return r.value
+
+
+class _PinnedBlock:
+    """Owner of one cufe_host_alloc block; freed when the last array using it goes."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        _check(lib().cufe_host_alloc(nbytes, C.byref(p)), "host_alloc")
+        self.ptr = p.value
+
+    def __del__(self):
+        try:
+            lib().cufe_host_free(self.ptr)
+        except Exception:
+            pass
+
+
+def host_empty(shape, dtype=np.uint32) -> np.ndarray:
+    """Uninitialised array in pinned host memory (server requests inside it are
+    copied to the GPU by DMA without staging)."""
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape))
+    owner = _PinnedBlock(max(1, count * dt.itemsize))
+    cbuf = (C.c_uint8 * (count * dt.itemsize)).from_address(owner.ptr)
+    cbuf._owner = owner
+    return np.frombuffer(cbuf, dtype=dt, count=count).reshape(shape)
 
 
 def autotune_report() -> str:

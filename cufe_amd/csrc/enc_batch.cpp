@@ -243,19 +243,23 @@ extern "C" int cufe_encbatch_run(cufe_encbatch *eb, const uint32_t *x, const uns
     }
     HIP_OK(hipEventRecord(eb->ev1, st));
     for (auto &g : eb->groups) {
-        int overflow = 0;
-        if (g->plan.check(st, &overflow)) return -1;
-        // a short window: redo this group eagerly at wider windows
-        for (int level = g->plan.level; overflow;) {
-            if (++level > 6) {
+        int overflow = 0, need_full = 0;
+        if (g->plan.check(st, &overflow, &need_full)) return -1;
+        // a short window or a chunk that needed every AES block: redo this
+        // group eagerly (wider windows / all blocks) and its ciphertexts
+        int level = g->plan.level;
+        while (overflow || need_full) {
+            if (overflow && ++level > 6) {
                 ipfe_set_error("Gaussian sampler: speculative window exhausted");
                 return -1;
             }
+            const int full = g->plan.full | need_full;
             std::vector<StreamReq> reqs = eb->requests(g->b0, g->b1, seed2, seed3, nonce0);
-            if (g->plan.build(reqs.data(), (int)reqs.size(), level) || g->plan.alloc_scratch() ||
-                g->plan.upload(st) || g->plan.launch(st) || g->plan.check(st, &overflow))
-                return -1;
-            if (!overflow && eb->enc_group(*g, dev)) return -1;
+            if (g->plan.build(reqs.data(), (int)reqs.size(), level) || g->plan.alloc_scratch()) return -1;
+            g->plan.full = full;
+            if (g->plan.upload(st) || g->plan.launch(st) || g->plan.check(st, &overflow, &need_full)) return -1;
+            if (need_full && full) need_full = 0;  // already ran with every block
+            if (!overflow && !need_full && eb->enc_group(*g, dev)) return -1;
             eb->redone++;
             if (eb->exec) {
                 (void)hipGraphExecDestroy(eb->exec);  // its kernels hold the old tables

@@ -17,6 +17,8 @@
 //      residues to its call's polynomial.
 #include <algorithm>
 
+#include <stdlib.h>
+
 #include "aes_dev.h"
 #include "kernels.h"
 
@@ -99,7 +101,19 @@ struct Cand {
     bool acc;
 };
 
-__device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int which, bool valid, int *status)
+// AES blocks of a chunk computed by default: 0..15 (bytes 0..255) and 17
+// (bytes 272..287, holding the sign byte 280).  Candidates read bytes 0..199
+// (CDT and Bernoulli words), the uniform words from byte 200 and byte 280; the
+// uniform scan stays below byte 256 unless it needs more than 14 words (seven
+// or more rejections, about 1e-11 per chunk).  Such a chunk sets status bit 4
+// and the caller reruns the group with all 24 blocks.
+constexpr int PART_BLOCKS = 17;
+__device__ __forceinline__ int chunk_block(int j, int full) { return (full || j < 16) ? j : 17; }
+
+// ulim: uniform words a partial chunk may scan (14 covers its blocks; lower
+// values exist to exercise the rerun)
+__device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int which, bool valid, int *status,
+                                               int full, int ulim)
 {
     const uint64_t K = c_K[which], BF = c_BF[which];
     const uint32_t UQ = (uint32_t)c_UQ[which];
@@ -113,8 +127,13 @@ __device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int wh
     int pos = c;
     if ((mask & 0xffu) != 0xffu) {
         int seen = 0;
+        bool partial_miss = false;
         pos = -1;
         for (int u = 0; u < 46; ++u) {
+            if (!full && u >= ulim) {
+                partial_miss = true;  // word 64 on lies in a block not computed
+                break;
+            }
             const bool kept = (u < 24) ? ((mask >> u) & 1u) : (ch[50 + u] < UQ);
             if (kept) {
                 if (seen == c) { pos = u; break; }
@@ -122,7 +141,7 @@ __device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int wh
             }
         }
         if (pos < 0) {
-            if (valid) atomicOr(status, 2);
+            if (valid) atomicOr(status, partial_miss ? 4 : 2);
             pos = 0;
         }
     }
@@ -151,30 +170,41 @@ __device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int wh
 
 // AES of nch chunks of one stream into LDS: chunk k at ks + 96 k uses
 // counters 24 (chunk_of(k)) .. +23.
-template <typename ChunkOf>
+template <bool REP, int NT, typename ChunkOf>
 __device__ __forceinline__ void aes_chunks(const GaussStream *S, const uint32_t *te, int nch, ChunkOf chunk_of,
-                                           uint32_t *ks)
+                                           uint32_t *ks, int full)
 {
-    for (int g = threadIdx.x; g < nch * 24; g += CAND_THREADS) {
-        const int k = g / 24, blk = g - 24 * k;
+    const int nb = full ? 24 : PART_BLOCKS;
+    for (int g = threadIdx.x; g < nch * nb; g += NT) {
+        const int k = g / nb, blk = chunk_block(g - nb * k, full);
         const uint32_t ctr = (uint32_t)(chunk_of(k) * 24 + blk);
         uint32_t o[4];
-        aes256_ctr_block(S->rk, te, S->nonce, ctr, o);
+        if (REP)
+            aes256_ctr_block_rep(S->rk, te, S->nonce, ctr, o);
+        else
+            aes256_ctr_block(S->rk, te, S->nonce, ctr, o);
         uint32_t *d = ks + k * CHUNK_WORDS + blk * 4;
         d[0] = o[0]; d[1] = o[1]; d[2] = o[2]; d[3] = o[3];
     }
 }
 
-__global__ __launch_bounds__(CAND_THREADS) void k_gauss_candidates(const GaussStream *__restrict__ streams, int nstreams,
+// REP: bank-replicated T-table (32 KiB of LDS) instead of the four 1 KiB
+// tables, shared by NT threads (512 with the replicated table, so that two
+// workgroups per CU still hold 16 waves).
+template <bool REP, int NT>
+__global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__restrict__ streams, int nstreams,
                                                                     const long long *__restrict__ tile_prefix,
                                                                     long long ntiles,
                                                                     const uint32_t *__restrict__ te_global,
                                                                     uint8_t *__restrict__ cnt, uint64_t *__restrict__ vals,
-                                                                    int *__restrict__ status)
+                                                                    int *__restrict__ status, int full, int ulim)
 {
-    __shared__ uint32_t te[AES_TABLE_WORDS];
+    __shared__ uint32_t te[REP ? AES_REP_WORDS : AES_TABLE_WORDS];
     __shared__ uint32_t ks[TILE_CHUNKS * CHUNK_WORDS];
-    aes_stage_tables(te, te_global);
+    if (REP)
+        aes_stage_tables_rep(te, te_global);
+    else
+        aes_stage_tables(te, te_global);
     const int sh = (threadIdx.x & 63) & 56;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int si = __builtin_amdgcn_readfirstlane(find_stream(tile_prefix, nstreams, tile));
@@ -184,12 +214,12 @@ __global__ __launch_bounds__(CAND_THREADS) void k_gauss_candidates(const GaussSt
         const int which = S->which;
         const bool store_vals = !S->coef0_only;
         __syncthreads();  // tables staged / previous tile's reads of ks done
-        aes_chunks(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks);
+        aes_chunks<REP, NT>(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks, full);
         __syncthreads();
-        for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += CAND_THREADS) {
+        for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += NT) {
             const int k = slot >> 3, c = slot & 7;
             const bool valid = k < nch;
-            const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status);
+            const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status, full, ulim);
             const uint64_t am = __ballot(cd.acc);
             const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
             if (valid) {
@@ -211,7 +241,7 @@ __global__ __launch_bounds__(CAND_THREADS) void k_gauss_firsts(const GaussStream
                                                                 const long long *__restrict__ first_prefix,
                                                                 const long long *__restrict__ firsts, long long nitems,
                                                                 const uint32_t *__restrict__ te_global,
-                                                                int *__restrict__ status)
+                                                                int *__restrict__ status, int full, int ulim)
 {
     __shared__ uint32_t te[AES_TABLE_WORDS];
     __shared__ uint32_t ks[FIRST_TILE * CHUNK_WORDS];
@@ -232,8 +262,9 @@ __global__ __launch_bounds__(CAND_THREADS) void k_gauss_firsts(const GaussStream
     }
     __syncthreads();
     // AES: 24 blocks per item, each item possibly from a different stream
-    for (int g = threadIdx.x; g < FIRST_TILE * 24; g += CAND_THREADS) {
-        const int k = g / 24, blk = g - 24 * k;
+    const int nb = full ? 24 : PART_BLOCKS;
+    for (int g = threadIdx.x; g < FIRST_TILE * nb; g += CAND_THREADS) {
+        const int k = g / nb, blk = chunk_block(g - nb * k, full);
         const int si = s_stream[k];
         if (si < 0) continue;
         const GaussStream *S = streams + si;
@@ -248,7 +279,7 @@ __global__ __launch_bounds__(CAND_THREADS) void k_gauss_firsts(const GaussStream
     const int si = s_stream[k];
     const bool valid = si >= 0;
     const int which = valid ? streams[si].which : 1;
-    const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status);
+    const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status, full, ulim);
     const uint64_t am = __ballot(cd.acc && valid);
     const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
     if (valid && cd.acc && (gm & ((1u << c) - 1u)) == 0) {
@@ -420,13 +451,40 @@ __global__ void k_aes_ctr(const uint32_t *__restrict__ rk, const uint32_t *__res
 }
 
 // ---------------------------------------------------------------------------
+// 0: four T-tables, 256 threads; 1 (default): replicated table, 512 threads;
+// 2: replicated table, 256 threads (CUFE_AES_REP)
+static int aes_rep_mode()
+{
+    const char *e = getenv("CUFE_AES_REP");
+    return e && *e ? atoi(e) : 1;
+}
+
+// Uniform words a partial chunk may scan before it asks for all 24 blocks:
+// 14 (all its blocks hold); CUFE_GAUSS_SCAN in [8, 14] lowers it so tests can
+// drive the rerun path.
+static int scan_limit()
+{
+    const char *e = getenv("CUFE_GAUSS_SCAN");
+    const int v = e && *e ? atoi(e) : 14;
+    return v < 8 ? 8 : (v > 14 ? 14 : v);
+}
+
 int launch_gauss_candidates(const GaussStream *d_streams, int nstreams, const long long *d_tile_prefix,
-                            long long ntiles, uint8_t *cnt, uint64_t *vals, int *status, hipStream_t st)
+                            long long ntiles, uint8_t *cnt, uint64_t *vals, int *status, int full, hipStream_t st)
 {
     if (ntiles <= 0) return 0;
     const unsigned grid = (unsigned)std::min<long long>(ntiles, 1ll << 20);
-    hipLaunchKernelGGL(k_gauss_candidates, dim3(grid), dim3(CAND_THREADS), 0, st, d_streams, nstreams, d_tile_prefix,
-                       ntiles, g_aes_te, cnt, vals, status);
+    const int rep = aes_rep_mode();
+    if (rep == 2)
+        hipLaunchKernelGGL((k_gauss_candidates<true, 256>), dim3(grid), dim3(256), 0, st, d_streams, nstreams,
+                           d_tile_prefix, ntiles, g_aes_te, cnt, vals, status, full, scan_limit());
+    else if (rep)
+        hipLaunchKernelGGL((k_gauss_candidates<true, 512>), dim3(grid), dim3(512), 0, st, d_streams, nstreams,
+                           d_tile_prefix, ntiles, g_aes_te, cnt, vals, status, full, scan_limit());
+    else
+        hipLaunchKernelGGL((k_gauss_candidates<false, CAND_THREADS>), dim3(grid), dim3(CAND_THREADS), 0, st,
+                           d_streams, nstreams, d_tile_prefix, ntiles, g_aes_te, cnt, vals, status, full,
+                           scan_limit());
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -454,11 +512,11 @@ int launch_gauss_scatter(const GaussStream *d_streams, int nstreams, const long 
 }
 
 int launch_gauss_firsts(const GaussStream *d_streams, int nstreams, const long long *d_first_prefix,
-                        const long long *firsts, long long nitems, int *status, hipStream_t st)
+                        const long long *firsts, long long nitems, int *status, int full, hipStream_t st)
 {
     if (nitems <= 0) return 0;
     hipLaunchKernelGGL(k_gauss_firsts, dim3((unsigned)((nitems + FIRST_TILE - 1) / FIRST_TILE)), dim3(CAND_THREADS), 0,
-                       st, d_streams, nstreams, d_first_prefix, firsts, nitems, g_aes_te, status);
+                       st, d_streams, nstreams, d_first_prefix, firsts, nitems, g_aes_te, status, full, scan_limit());
     HIP_OK(hipGetLastError());
     return 0;
 }

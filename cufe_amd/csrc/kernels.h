@@ -65,15 +65,17 @@ struct GaussStream {
 
 // Candidate pass: for every chunk of every stream, AES + CDT + Bernoulli;
 // writes the accepted count and the accepted (z | sign << 63) values in order.
+// full: compute all 24 AES blocks of each chunk (else the 17 that candidates
+// read unless a chunk's uniform scan runs long, which sets status bit 4)
 int launch_gauss_candidates(const GaussStream *d_streams, int nstreams, const long long *d_tile_prefix,
-                            long long ntiles, uint8_t *cnt, uint64_t *vals, int *status, hipStream_t st);
+                            long long ntiles, uint8_t *cnt, uint64_t *vals, int *status, int full, hipStream_t st);
 // Walk: per stream, the call boundaries from the counts; writes per chunk
 // (call << 13 | offset) or -1.  status[stream] = 1 when the window was short.
 int launch_gauss_walk(const GaussStream *d_streams, int nstreams, const uint8_t *cnt, int *pos, long long *firsts,
                       int *status, hipStream_t st);
 // Compact streams: recompute the first non-empty chunk of every call and write sample 0.
 int launch_gauss_firsts(const GaussStream *d_streams, int nstreams, const long long *d_first_prefix,
-                        const long long *firsts, long long nitems, int *status, hipStream_t st);
+                        const long long *firsts, long long nitems, int *status, int full, hipStream_t st);
 int launch_gauss_scatter(const GaussStream *d_streams, int nstreams, const long long *d_chunk_prefix,
                          long long nchunks_total, const uint8_t *cnt, const uint64_t *vals, const int *pos,
                          hipStream_t st);

@@ -8,6 +8,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 
 namespace ipfe {
@@ -206,16 +207,16 @@ int GaussPlan::launch(hipStream_t st)
 {
     if (launch_fill_i32(s_pos, -1, (size_t)full_chunks, st) || launch_fill_i32(d_status.get(), 0, n + 1, st))
         return -1;
-    if (launch_gauss_candidates(d_streams.get(), n, d_tp.get(), tiles, s_cnt, s_vals, d_status.get() + n, st))
+    if (launch_gauss_candidates(d_streams.get(), n, d_tp.get(), tiles, s_cnt, s_vals, d_status.get() + n, full, st))
         return -1;
     if (launch_gauss_walk(d_streams.get(), n, s_cnt, s_pos, d_first.get(), d_status.get(), st)) return -1;
     if (launch_gauss_scatter(d_streams.get(), n, d_cp.get(), full_chunks, s_cnt, s_vals, s_pos, st)) return -1;
-    if (launch_gauss_firsts(d_streams.get(), n, d_fp.get(), d_first.get(), nfirst, d_status.get() + n, st))
+    if (launch_gauss_firsts(d_streams.get(), n, d_fp.get(), d_first.get(), nfirst, d_status.get() + n, full, st))
         return -1;
     return 0;
 }
 
-int GaussPlan::check(hipStream_t st, int *overflow)
+int GaussPlan::check(hipStream_t st, int *overflow, int *need_full)
 {
     std::vector<int> status(n + 1);
     HIP_OK(hipMemcpyAsync(status.data(), d_status.get(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
@@ -224,6 +225,7 @@ int GaussPlan::check(hipStream_t st, int *overflow)
         ipfe_set_error("Gaussian sampler: a chunk needed more than 46 uniform words (reference reads past its buffer)");
         return -1;
     }
+    *need_full = (status[n] & 4) ? 1 : 0;
     *overflow = 0;
     for (int i = 0; i < n; ++i) *overflow |= status[i];
     return 0;
@@ -232,8 +234,13 @@ int GaussPlan::check(hipStream_t st, int *overflow)
 static int run_gauss_group(const StreamReq *reqs, int n, int level, hipStream_t st, int *overflow)
 {
     GaussPlan p;
-    if (p.build(reqs, n, level) || p.alloc_scratch() || p.upload(st) || p.launch(st)) return -1;
-    return p.check(st, overflow);
+    if (p.build(reqs, n, level) || p.alloc_scratch()) return -1;
+    for (;;) {
+        int need_full = 0;
+        if (p.upload(st) || p.launch(st) || p.check(st, overflow, &need_full)) return -1;
+        if (!need_full || p.full) return 0;
+        p.full = 1;  // a chunk needed the AES blocks the default set omits
+    }
 }
 
 int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch_budget)
@@ -268,3 +275,53 @@ int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch
 }
 
 }  // namespace ipfe
+
+// ---------------------------------------------------------------------------
+// Pinned host memory handed out by the library (hipHostMalloc), registered so
+// that the server can DMA request buffers that lie inside it directly.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_pin_mu;
+std::map<uintptr_t, size_t> g_pinned;  // start -> bytes
+}  // namespace
+
+namespace ipfe {
+bool host_pinned(const void *p, size_t bytes)
+{
+    const uintptr_t a = (uintptr_t)p;
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    auto it = g_pinned.upper_bound(a);
+    if (it == g_pinned.begin()) return false;
+    --it;
+    return a >= it->first && a + bytes <= it->first + it->second;
+}
+}  // namespace ipfe
+
+extern "C" int cufe_host_alloc(size_t bytes, void **out)
+{
+    if (!out || bytes == 0) {
+        ipfe_set_error("cufe_host_alloc: bad arguments");
+        return -1;
+    }
+    if (!ipfe::device()) return -1;
+    void *p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        ipfe_set_error("hipHostMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pinned[(uintptr_t)p] = bytes;
+    *out = p;
+    return 0;
+}
+
+extern "C" void cufe_host_free(void *p)
+{
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        g_pinned.erase((uintptr_t)p);
+    }
+    (void)hipHostFree(p);
+}

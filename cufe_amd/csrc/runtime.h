@@ -78,6 +78,7 @@ int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch
 // status check that reports a speculative window that was too short.
 struct GaussPlan {
     int n = 0, level = 0;
+    int full = 0;  // all 24 AES blocks per chunk (set after a chunk needed them)
     long long tiles = 0, chunks = 0, full_chunks = 0, nfirst = 0;
     std::vector<GaussStream> hs;
     std::vector<long long> tile_prefix, chunk_prefix, first_prefix;
@@ -100,7 +101,9 @@ struct GaussPlan {
     void rekey(const StreamReq *reqs);
     int upload(hipStream_t st);
     int launch(hipStream_t st);
-    int check(hipStream_t st, int *overflow);
+    // overflow: a window was short; need_full: a chunk needed AES blocks the
+    // default set leaves out (rerun with full = 1)
+    int check(hipStream_t st, int *overflow, int *need_full);
 };
 
 // Speculative window (chunks) for a stream of ncalls calls of sampler `which`
@@ -109,5 +112,8 @@ long long gauss_window(int which, int ncalls, int m);
 size_t gauss_bytes(int which, int ncalls, int coef0_only);
 
 int sync_stream(hipStream_t st);
+
+// [p, p + bytes) lies inside one block from cufe_host_alloc (pinned).
+bool host_pinned(const void *p, size_t bytes);
 
 }  // namespace ipfe

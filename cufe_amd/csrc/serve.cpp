@@ -116,7 +116,6 @@ int launch_batch(cufe_server *s, Slot &sl)
     const int mode = contract_mode();
     // layout: requests grouped by model in order of first appearance, each
     // group starting on a 32-row boundary (the digit store's row blocks)
-    std::vector<int> order;
     std::vector<const Model *> gm;
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         const Model *m = sl.reqs[i]->m;
@@ -144,20 +143,32 @@ int launch_batch(cufe_server *s, Slot &sl)
         ipfe_set_error("server: batch layout of %d rows exceeds the slot capacity %d", row, s->cap);
         return -1;
     }
-    // host staging: c_L rows in slot order, then each request's c0 packed
+    // upload: c_L rows to their slot rows, each request's c0 packed after
+    // them.  Buffers in pinned memory from cufe_host_alloc are copied by DMA
+    // directly; others are first copied into the slot's pinned staging.
     uint32_t *h_c0 = sl.h_stage + (size_t)s->cap * kPoly;
+    uint32_t *d_c0 = sl.d_stage.get() + (size_t)s->cap * kPoly;
     std::vector<size_t> c0_off(sl.reqs.size());
     size_t c0_words = 0;
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         const Req &r = *sl.reqs[i];
-        memcpy(sl.h_stage + (size_t)sl.req_row[i] * kPoly, r.cl, (size_t)r.n * kPoly * 4);
+        const size_t cl_bytes = (size_t)r.n * kPoly * 4, c0_bytes = (size_t)r.n * r.l * 12;
+        const size_t cl_at = (size_t)sl.req_row[i] * kPoly;
+        const uint32_t *src = r.cl;
+        if (!host_pinned(r.cl, cl_bytes)) {
+            memcpy(sl.h_stage + cl_at, r.cl, cl_bytes);
+            src = sl.h_stage + cl_at;
+        }
+        HIP_OK(hipMemcpyAsync(sl.d_stage.get() + cl_at, src, cl_bytes, hipMemcpyHostToDevice, sl.st));
         c0_off[i] = c0_words;
-        memcpy(h_c0 + c0_words, r.c0, (size_t)r.n * r.l * 12);
+        src = r.c0;
+        if (!host_pinned(r.c0, c0_bytes)) {
+            memcpy(h_c0 + c0_words, r.c0, c0_bytes);
+            src = h_c0 + c0_words;
+        }
+        HIP_OK(hipMemcpyAsync(d_c0 + c0_words, src, c0_bytes, hipMemcpyHostToDevice, sl.st));
         c0_words += (size_t)r.n * r.l * 3;
     }
-    uint32_t *d_c0 = sl.d_stage.get() + (size_t)s->cap * kPoly;
-    HIP_OK(hipMemcpyAsync(sl.d_stage.get(), sl.h_stage, (size_t)row * kPoly * 4, hipMemcpyHostToDevice, sl.st));
-    HIP_OK(hipMemcpyAsync(d_c0, h_c0, c0_words * 4, hipMemcpyHostToDevice, sl.st));
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         const Req &r = *sl.reqs[i];
         if (sl.rows.put(sl.req_row[i], r.n, sl.d_stage.get() + (size_t)sl.req_row[i] * kPoly, d_c0 + c0_off[i], sl.st,
@@ -428,6 +439,15 @@ extern "C" int cufe_server_start(cufe_server *s)
         HIP_OK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
     }
     HIP_OK(hipDeviceSynchronize());
+    // Run every model once per power-of-two batch size on the (zeroed) rows of
+    // slot 0, so the contraction's block shapes are tuned here and not on the
+    // first requests.
+    Slot &w = s->slots[0];
+    const int mode = contract_mode();
+    for (auto &m : s->models)
+        for (int R = 1; R <= s->cap; R *= 2)
+            if (predict_rows(*m, w.rows, 0, R, w.sc, s->cap, mode, w.h_labels, nullptr, nullptr, w.st)) return -1;
+    HIP_OK(hipStreamSynchronize(w.st));
     s->started = true;
     s->worker = std::thread(worker_main, s);
     return 0;

@@ -256,6 +256,11 @@ int cufe_server_replay(cufe_server *srv, int nreq, const double *arrival_s, cons
 int cufe_server_stats(cufe_server *srv, long long *out8);
 /* Completes queued requests, stops the worker, frees device memory. */
 void cufe_server_destroy(cufe_server *srv);
+/* Pinned host memory (hipHostMalloc).  Request rows that lie inside such a
+ * block are copied to the device by DMA directly; other buffers go through
+ * the server's pinned staging first. */
+int cufe_host_alloc(size_t bytes, void **out);
+void cufe_host_free(void *p);
 
 #ifdef __cplusplus
 }

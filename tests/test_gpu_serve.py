@@ -149,6 +149,24 @@ def test_one_request_per_batch(gpu, env):
         srv.close()
 
 
+def test_pinned_requests(gpu, env):
+    """Rows in pinned host memory are copied by DMA directly: same results."""
+    srv = start_server(gpu, env, max_batch_rows=256)
+    try:
+        reqs = []
+        for mid, r0, n in ((0, 5, 40), (1, 0, 17), (3, 2, 20), (2, 30, 64)):
+            cl, c0 = request(env, mid, r0, n)
+            pcl, pc0 = gpu.host_empty(cl.shape), gpu.host_empty(c0.shape)
+            pcl[...] = cl
+            pc0[...] = c0
+            reqs.append((srv.submit(mid, pcl, pc0, want_kvalues=True), mid, r0, n))
+        for t, mid, r0, n in reqs:
+            lab, kv, _ = srv.wait(t, 60000)
+            check(env, mid, r0, n, lab, kv)
+    finally:
+        srv.close()
+
+
 def test_poisson_replay(gpu, env):
     srv = start_server(gpu, env, max_batch_rows=1024)
     try:

@@ -202,7 +202,8 @@ def time_ntt(npoly: int, inverse: bool, iters: int) -> float:
 
 
 def microbench(kind: int) -> float:
-    """0: fp64 FMA TFLOP/s, 1: int32 mul T ops/s, 2: copy GB/s, 3: AES G blocks/s."""
+    """0: fp64 FMA TFLOP/s, 1: int32 mul T ops/s, 2: copy GB/s, 3: AES G blocks/s,
+    4: AES (replicated table) G blocks/s, 5: int8 MFMA T ops/s, 6: fp64 MFMA TFLOP/s."""
     r = C.c_double()
     _check(lib().cufe_microbench(kind, C.byref(r)), "microbench")
     return r.value

@@ -89,6 +89,41 @@ __global__ __launch_bounds__(256) void k_mb_aes_rep(const uint32_t *__restrict__
     out[blockIdx.x * blockDim.x + threadIdx.x] = x;
 }
 
+typedef int mb_v4i __attribute__((ext_vector_type(4)));
+typedef int mb_v16i __attribute__((ext_vector_type(16)));
+typedef double mb_v4d __attribute__((ext_vector_type(4)));
+
+// Two independent v_mfma_i32_32x32x32_i8 chains per wave: the int8 matrix
+// rate with operands in registers.
+__global__ __launch_bounds__(256) void k_mb_mfma_i8(int *out, int iters)
+{
+    const mb_v4i a = {(int)threadIdx.x, 1, 2, 3}, b = {3, 2, 1, (int)threadIdx.x};
+    mb_v16i c0 = {}, c1 = {};
+    for (int it = 0; it < iters; ++it) {
+        c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c1, 0, 0, 0);
+    }
+    int x = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x ^= c0[r] ^ c1[r];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
+// Four independent v_mfma_f64_16x16x4_f64 chains per wave.
+__global__ __launch_bounds__(256) void k_mb_mfma_f64(double *out, int iters)
+{
+    const double a = threadIdx.x * 1e-3, b = 0.999;
+    mb_v4d c0 = {}, c1 = {}, c2 = {}, c3 = {};
+    for (int it = 0; it < iters; ++it) {
+        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c1, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c2, 0, 0, 0);
+        c3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c3, 0, 0, 0);
+    }
+    const mb_v4d sum = c0 + c1 + c2 + c3;
+    out[blockIdx.x * blockDim.x + threadIdx.x] = sum[0] + sum[1] + sum[2] + sum[3];
+}
+
 // Checks that both AES variants produce the same keystream words.
 __global__ void k_mb_aes_check(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global,
                                uint32_t *__restrict__ bad)
@@ -111,7 +146,8 @@ using namespace ipfe;
 
 // kind 0: fp64 FMA (TFLOP/s), 1: int32 mul_lo+mul_hi+add (T int-ops/s),
 // 2: device copy (GB/s of read+write), 3: AES-256 CTR blocks (G blocks/s),
-// 4: the same with the bank-replicated table (and an equality check).
+// 4: the same with the bank-replicated table (and an equality check),
+// 5: int8 MFMA 32x32x32 (T ops/s), 6: fp64 MFMA 16x16x4 (TFLOP/s).
 extern "C" int cufe_microbench(int kind, double *result)
 {
     Device *dev = device();
@@ -125,7 +161,24 @@ extern "C" int cufe_microbench(int kind, double *result)
     const int blocks = cus * 8;
     float ms = 0;
     double work = 0;
-    if (kind == 0 || kind == 1 || kind == 3 || kind == 4) {
+    if (kind == 5 || kind == 6) {
+        DevBuf<double> out;
+        if (out.alloc((size_t)blocks * 256)) return -1;
+        const int iters = 4096;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_OK(hipEventRecord(a, st));
+            if (kind == 5)
+                hipLaunchKernelGGL(k_mb_mfma_i8, dim3(blocks), dim3(256), 0, st, (int *)out.get(), iters);
+            else
+                hipLaunchKernelGGL(k_mb_mfma_f64, dim3(blocks), dim3(256), 0, st, out.get(), iters);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipEventRecord(b, st));
+            HIP_OK(hipEventSynchronize(b));
+        }
+        HIP_OK(hipEventElapsedTime(&ms, a, b));
+        const double waves = (double)blocks * 4;
+        work = kind == 5 ? waves * iters * 2 * (32.0 * 32 * 32 * 2) : waves * iters * 4 * (16.0 * 16 * 4 * 2);
+    } else if (kind == 0 || kind == 1 || kind == 3 || kind == 4) {
         DevBuf<uint32_t> out;
         if (out.alloc((size_t)blocks * 256 * 2)) return -1;
         const int iters = 4096;
@@ -189,6 +242,6 @@ extern "C" int cufe_microbench(int kind, double *result)
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     const double s = ms * 1e-3;
-    *result = (kind == 0 || kind == 1) ? work / s / 1e12 : (kind == 2 ? work / s / 1e9 : work / s / 1e9);
+    *result = (kind == 0 || kind == 1 || kind >= 5) ? work / s / 1e12 : work / s / 1e9;
     return 0;
 }

@@ -119,7 +119,9 @@ int cufe_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r);
 
 /* Measured ceilings of this GPU: kind 0 = fp64 FMA (TFLOP/s), 1 = 32-bit
  * mul_lo + mul_hi + add (T int-ops/s), 2 = device copy (GB/s read + write),
- * 3 = AES-256-CTR keystream (G blocks/s). */
+ * 3 = AES-256-CTR keystream (G blocks/s), 4 = the same with the
+ * bank-replicated table, 5 = int8 MFMA 32x32x32 (T ops/s), 6 = fp64 MFMA
+ * 16x16x4 (TFLOP/s). */
 int cufe_microbench(int kind, double *result);
 
 /* The scheme at a vector length l <= 785 (uneven-length batching).  With

@@ -63,6 +63,7 @@ def lib() -> C.CDLL:
         "cufe_microbench": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
         "cufe_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
         "cufe_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+        "cufe_contract_variants": (C.c_int, [C.c_int, C.POINTER(C.c_uint)]),
         "cufe_host_free": (None, [C.c_void_p]),
         "rlwe_sife_setup_gpu": (None, [_u32p, _u32p, _u8p, _u8p, C.POINTER(C.c_float)]),
         "rlwe_sife_encrypt_gpu": (None, [_u32p, _u32p, _u32p, _u8p, _u8p, C.c_int, C.POINTER(C.c_float)]),
@@ -233,6 +234,13 @@ def host_empty(shape, dtype=np.uint32) -> np.ndarray:
     cbuf = (C.c_uint8 * (count * dt.itemsize)).from_address(owner.ptr)
     cbuf._owner = owner
     return np.frombuffer(cbuf, dtype=dt, count=count).reshape(shape)
+
+
+def contract_variants(nb: int) -> list:
+    """Indices of the int8 contraction variants that run with nb key digits."""
+    m = C.c_uint()
+    n = lib().cufe_contract_variants(nb, C.byref(m))
+    return [v for v in range(n) if (m.value >> v) & 1]
 
 
 def autotune_report() -> str:

@@ -144,6 +144,243 @@ __global__ __launch_bounds__(64 * WR * WK, (i8_min_waves<WR, WK, NB>())) void k_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined variant: the digit fragments go from global memory straight into
+// an LDS ring of STAGES k steps (global_load_lds_dwordx4: lane l's 16 bytes
+// land at the wave's LDS base + 16 l, which is the fragment layout), so loads
+// for the next STAGES - 1 k steps are in flight while a step computes, and no
+// registers hold them.  Every wave issues the same number of fragment loads
+// per step (F = 4 WR + NB WK fragments over WR WK waves), so one
+// s_waitcnt vmcnt(loads issued after step ks) and a barrier make step ks
+// visible to all waves.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+template <int WR, int WK, int NB, int STAGES, int KPB>
+__global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *__restrict__ A,
+                                                                     const v4i *__restrict__ B, int R, int S,
+                                                                     int ksteps, long long a_stride,
+                                                                     long long b_stride, uint32_t q,
+                                                                     uint32_t *__restrict__ D, int ntk, int ntiles)
+{
+    constexpr int W = WR * WK;
+    constexpr int F = WR * DG_A + WK * NB;  // fragments per k step
+    static_assert(F % W == 0, "fragments must split evenly over the waves");
+    constexpr int FW = F / W;  // fragment loads per wave per k step
+    constexpr int STEP_V4 = F * FRAG_V4;
+    constexpr int SLOT_V4 = KPB * STEP_V4;  // a ring slot holds KPB k steps
+    __shared__ v4i ring[STAGES * SLOT_V4];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int per = gridDim.x / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (t >= ntiles) return;
+    const int tk = t % ntk, tr = t / ntk;
+    const int wr = w / WK, wk = w % WK;
+    // this wave's fragments: f = w + u W
+    const v4i *src[FW];
+    long long step[FW];
+#pragma unroll
+    for (int u = 0; u < FW; ++u) {
+        const int f = w + u * W;
+        if (f < WR * DG_A) {
+            src[u] = A + (size_t)(tr * WR + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4 + lane;
+            step[u] = DG_A * FRAG_V4;
+        } else {
+            const int fb = f - WR * DG_A;
+            src[u] = B + (size_t)(tk * WK + fb / NB) * b_stride + (fb % NB) * FRAG_V4 + lane;
+            step[u] = NB * FRAG_V4;
+        }
+    }
+    const int ngroups = (ksteps + KPB - 1) / KPB;
+    // group g = k steps [g KPB, g KPB + KPB) into ring slot g mod STAGES
+    auto issue = [&](int g) {
+        v4i *slot = ring + (g % STAGES) * SLOT_V4;
+#pragma unroll
+        for (int j = 0; j < KPB; ++j) {
+            const int ks = g * KPB + j;
+            if (ks >= ksteps) break;
+#pragma unroll
+            for (int u = 0; u < FW; ++u) {
+                const int f = w + u * W;
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(src[u] + (size_t)ks * step[u]),
+                                                 (lds_void_t *)(slot + j * STEP_V4 + f * FRAG_V4), 16, 0, 0);
+            }
+        }
+    };
+    v16i acc[DG_A][NB];
+#pragma unroll
+    for (int d = 0; d < DG_A; ++d)
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+#pragma unroll
+    for (int p = 0; p < STAGES - 1; ++p)
+        if (p < ngroups) issue(p);
+    for (int g = 0; g < ngroups; ++g) {
+        // loads issued after group g: the next STAGES - 2 groups, all full
+        // unless they include the last group (then wait for everything)
+        if (g + STAGES - 2 < ngroups - 1)
+            wait_vmcnt<(STAGES - 2) * FW * KPB>();
+        else
+            wait_vmcnt<0>();
+        // group g in LDS for every wave; slot (g - 1) read by all.  A bare
+        // barrier: a workgroup fence would also wait for the later groups'
+        // loads.  The LDS reads of group g - 1 completed before their MFMAs.
+        asm volatile("s_barrier" ::: "memory");
+        if (g + STAGES - 1 < ngroups) issue(g + STAGES - 1);
+        const v4i *slot = ring + (g % STAGES) * SLOT_V4;
+#pragma unroll
+        for (int j = 0; j < KPB; ++j) {
+            if (g * KPB + j >= ksteps) break;
+            const v4i *st = slot + j * STEP_V4;
+            v4i fa[DG_A], fb[NB];
+#pragma unroll
+            for (int d = 0; d < DG_A; ++d) fa[d] = st[(wr * DG_A + d) * FRAG_V4 + lane];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) fb[e] = st[(WR * DG_A + wk * NB + e) * FRAG_V4 + lane];
+#pragma unroll
+            for (int d = 0; d < DG_A; ++d)
+#pragma unroll
+                for (int e = 0; e < NB; ++e)
+                    acc[d][e] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d], fb[e], acc[d][e], 0, 0, 0);
+        }
+    }
+    const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
+    if (key >= S) return;
+    const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
+    const double qinv = 1.0 / (double)q;
+    const long long qq = q;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = row0 + (i >> 2) * 8 + (i & 3);
+        if (row >= R) continue;
+        long long tot = 0;
+#pragma unroll
+        for (int e = NB - 1; e >= 0; --e) {
+            long long T = 0;
+#pragma unroll
+            for (int d = 0; d < DG_A; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
+            long long m = T - (long long)floor((double)T * qinv) * qq;
+            m += (m < 0) ? qq : 0;
+            m -= (m >= qq) ? qq : 0;
+            tot = tot * 256 + m;
+            tot -= (long long)floor((double)tot * qinv) * qq;
+            tot += (tot < 0) ? qq : 0;
+            tot -= (tot >= qq) ? qq : 0;
+        }
+        D[(size_t)row * S + key] = (uint32_t)tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hybrid variant: one row block shared by WK waves, one key block per wave.
+// The row digits (4 fragments per k step, used by every wave) go through the
+// LDS ring by global_load_lds; each wave's key digits (used by that wave only)
+// are loaded straight into registers, STAGES - 1 steps ahead.  LDS then
+// carries 4 KB per wave and step instead of 4 + NB KB, which keeps it below
+// the MFMA time.  Loads of step j are issued together (row fragment DMA first,
+// then the key loads), so s_waitcnt vmcnt counts whole steps.
+// ---------------------------------------------------------------------------
+template <int WK, int NB, int STAGES>
+__global__ __launch_bounds__(64 * WK, 2) void k_contract_i8_hyb(const v4i *__restrict__ A,
+                                                               const v4i *__restrict__ B, int R, int S,
+                                                               int ksteps, long long a_stride, long long b_stride,
+                                                               uint32_t q, uint32_t *__restrict__ D, int ntk,
+                                                               int ntiles)
+{
+    static_assert(DG_A % WK == 0, "row fragments must split evenly over the waves");
+    constexpr int FA = DG_A / WK;  // row fragment loads per wave per step
+    constexpr int P = STAGES - 1;  // steps in flight
+    constexpr int SLOT_V4 = DG_A * FRAG_V4;
+    __shared__ v4i ring[STAGES * SLOT_V4];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int per = gridDim.x / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (t >= ntiles) return;
+    const int tk = t % ntk, tr = t / ntk;
+    const v4i *asrc = A + (size_t)tr * a_stride + lane;          // + (f + 4 ks) 64
+    const v4i *bsrc = B + (size_t)(tk * WK + w) * b_stride + lane;  // + (e + NB ks) 64
+    v4i breg[P][NB];
+    auto issue = [&](int ks, v4i (&bdst)[NB]) {
+        v4i *slot = ring + (ks % STAGES) * SLOT_V4;
+#pragma unroll
+        for (int u = 0; u < FA; ++u) {
+            const int f = w + u * WK;
+            __builtin_amdgcn_global_load_lds((glb_void_t *)(asrc + ((size_t)ks * DG_A + f) * FRAG_V4),
+                                             (lds_void_t *)(slot + f * FRAG_V4), 16, 0, 0);
+        }
+#pragma unroll
+        for (int e = 0; e < NB; ++e) bdst[e] = bsrc[((size_t)ks * NB + e) * FRAG_V4];
+    };
+    v16i acc[DG_A][NB];
+#pragma unroll
+    for (int d = 0; d < DG_A; ++d)
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+        if (p < ksteps) issue(p, breg[p]);
+    for (int ks0 = 0; ks0 < ksteps; ks0 += P) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int ks = ks0 + j;
+            if (ks >= ksteps) break;
+            // steps issued after ks: ks+1 .. ks+P-1 (FA + NB loads each)
+            if (ks + P - 1 < ksteps)
+                wait_vmcnt<(P - 1) * (FA + NB)>();
+            else
+                wait_vmcnt<0>();
+            asm volatile("s_barrier" ::: "memory");  // row fragments of step ks in LDS
+            const v4i *st = ring + (ks % STAGES) * SLOT_V4;
+            v4i fa[DG_A], fb[NB];
+#pragma unroll
+            for (int d = 0; d < DG_A; ++d) fa[d] = st[d * FRAG_V4 + lane];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) fb[e] = breg[j][e];
+            if (ks + P < ksteps) issue(ks + P, breg[j]);
+#pragma unroll
+            for (int d = 0; d < DG_A; ++d)
+#pragma unroll
+                for (int e = 0; e < NB; ++e)
+                    acc[d][e] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d], fb[e], acc[d][e], 0, 0, 0);
+        }
+    }
+    const int key = tk * (32 * WK) + w * 32 + (lane & 31);
+    if (key >= S) return;
+    const int row0 = tr * 32 + (lane >> 5) * 4;
+    const double qinv = 1.0 / (double)q;
+    const long long qq = q;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = row0 + (i >> 2) * 8 + (i & 3);
+        if (row >= R) continue;
+        long long tot = 0;
+#pragma unroll
+        for (int e = NB - 1; e >= 0; --e) {
+            long long T = 0;
+#pragma unroll
+            for (int d = 0; d < DG_A; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
+            long long m = T - (long long)floor((double)T * qinv) * qq;
+            m += (m < 0) ? qq : 0;
+            m -= (m >= qq) ? qq : 0;
+            tot = tot * 256 + m;
+            tot -= (long long)floor((double)tot * qinv) * qq;
+            tot += (tot < 0) ? qq : 0;
+            tot -= (tot >= qq) ? qq : 0;
+        }
+        D[(size_t)row * S + key] = (uint32_t)tot;
+    }
+}
+
 __device__ __forceinline__ int digit_split(int &x)
 {
     const int d = ((x + 128) & 255) - 128;
@@ -237,32 +474,54 @@ int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int
     return 0;
 }
 
-// Block shapes (row blocks x key blocks) the launcher can run.
-static const int kShapes[][2] = {{2, 4}, {2, 2}, {4, 2}, {1, 4}, {4, 1}};
-constexpr int kNumShapes = 5;
+// Kernel variants the launcher can run: row blocks x key blocks per block
+// tile, and the LDS ring depth of the pipelined kernel (0: the register-staged
+// double-buffered kernel).  A pipelined variant needs its fragments to split
+// evenly over its waves for the given key digits.
+// kpb: k steps per barrier of the pipelined kernel; -1 marks the hybrid
+// kernel (row digits through LDS, key digits straight to registers).
+struct I8Variant {
+    int wr, wk, stages, kpb;
+};
+static const I8Variant kVariants[] = {{2, 4, 0, 1}, {2, 2, 0, 1}, {4, 2, 0, 1}, {1, 4, 0, 1}, {4, 1, 0, 1},
+                                      {2, 2, 4, 1}, {1, 4, 4, 1}, {2, 4, 4, 1}, {2, 2, 3, 1}, {1, 4, 3, 1},
+                                      {2, 2, 2, 2}, {2, 2, 3, 2}, {1, 4, 2, 2}, {2, 4, 2, 2},
+                                      {1, 4, 3, -1}, {1, 4, 4, -1}, {1, 2, 3, -1}, {1, 4, 2, -1}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
-int i8_shape_count() { return kNumShapes; }
+int i8_shape_count() { return kNumVariants; }
 void i8_shape(int v, int *wr, int *wk)
 {
-    *wr = kShapes[v][0];
-    *wk = kShapes[v][1];
+    *wr = kVariants[v].wr;
+    *wk = kVariants[v].wk;
+}
+int i8_variant_stages(int v) { return kVariants[v].stages; }
+bool i8_variant_ok(int v, int nb)
+{
+    if (v < 0 || v >= kNumVariants || nb < 2 || nb > 3) return false;
+    const I8Variant &x = kVariants[v];
+    if (!x.stages) return true;
+    if (x.kpb < 0) return nb == 2 || v == 14;  // hybrid: all with two key digits, one with three
+    const int W = x.wr * x.wk, F = x.wr * DG_A + x.wk * nb;
+    if (F % W) return false;
+    if (nb == 3) return v == 6 || v == 9 || v == 12;  // the instantiated three-digit kernels
+    return true;
 }
 
 // A: row digit store of one limb at row r0 (r0 a multiple of 32; the store
 // holds 128 rows past its capacity), ksteps_a its k steps; B: key digit store
 // of one limb with nb digits, ksteps k steps (<= ksteps_a), keys padded to a
-// multiple of 128.  variant: index into the block shapes.
+// multiple of 128.  variant: index into kVariants.
 int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
                        uint32_t *D, hipStream_t st, int variant)
 {
     if (R <= 0 || S <= 0) return 0;
-    if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 17) || variant < 0 || variant >= kNumShapes ||
-        nb < 2 || nb > 3) {
+    if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 17) || !i8_variant_ok(variant, nb)) {
         ipfe_set_error("int8 contraction: %d k steps (row store %d), variant %d, %d key digits", ksteps, ksteps_a,
                        variant, nb);
         return -1;
     }
-    const int WR = kShapes[variant][0], WK = kShapes[variant][1];
+    const int WR = kVariants[variant].wr, WK = kVariants[variant].wk, ST = kVariants[variant].stages;
     // row tiles never reach past the 128 padding rows; key tiles stay within
     // the 128-key padding
     const int ntk = (S + 32 * WK - 1) / (32 * WK), ntr = (R + 32 * WR - 1) / (32 * WR);
@@ -274,13 +533,32 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
     const uint32_t q = qmod(s);
 #define CI8(wr, wk, n) \
     hipLaunchKernelGGL((k_contract_i8<wr, wk, n>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
+#define CI8D(wr, wk, n, stg, kpb)                                                                                  \
+    hipLaunchKernelGGL((k_contract_i8_dma<wr, wk, n, stg, kpb>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, \
+                       ntk, ntiles)
+#define CI8H(wk, n, stg)                                                                                             \
+    hipLaunchKernelGGL((k_contract_i8_hyb<wk, n, stg>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk,    \
+                       ntiles)
     if (nb == 2) {
         switch (variant) {
         case 0: CI8(2, 4, 2); break;
         case 1: CI8(2, 2, 2); break;
         case 2: CI8(4, 2, 2); break;
         case 3: CI8(1, 4, 2); break;
-        default: CI8(4, 1, 2); break;
+        case 4: CI8(4, 1, 2); break;
+        case 5: CI8D(2, 2, 2, 4, 1); break;
+        case 6: CI8D(1, 4, 2, 4, 1); break;
+        case 7: CI8D(2, 4, 2, 4, 1); break;
+        case 8: CI8D(2, 2, 2, 3, 1); break;
+        case 9: CI8D(1, 4, 2, 3, 1); break;
+        case 10: CI8D(2, 2, 2, 2, 2); break;
+        case 11: CI8D(2, 2, 2, 3, 2); break;
+        case 12: CI8D(1, 4, 2, 2, 2); break;
+        case 13: CI8D(2, 4, 2, 2, 2); break;
+        case 14: CI8H(4, 2, 3); break;
+        case 15: CI8H(4, 2, 4); break;
+        case 16: CI8H(2, 2, 3); break;
+        default: CI8H(4, 2, 2); break;  // variant 17
         }
     } else {
         switch (variant) {
@@ -288,10 +566,17 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         case 1: CI8(2, 2, 3); break;
         case 2: CI8(4, 2, 3); break;
         case 3: CI8(1, 4, 3); break;
-        default: CI8(4, 1, 3); break;
+        case 4: CI8(4, 1, 3); break;
+        case 6: CI8D(1, 4, 3, 4, 1); break;
+        case 9: CI8D(1, 4, 3, 3, 1); break;
+        case 12: CI8D(1, 4, 3, 2, 2); break;
+        default: CI8H(4, 3, 3); break;  // variant 14
         }
     }
 #undef CI8
+#undef CI8D
+#undef CI8H
+    (void)ST;
     HIP_OK(hipGetLastError());
     return 0;
 }

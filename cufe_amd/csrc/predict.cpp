@@ -69,9 +69,10 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
     if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, 0)) return -1;
     int best = 0;
     float best_ms = 1e30f;
-    char line[256];
+    char line[512];
     int off = snprintf(line, sizeof line, "S=%d R=%d ksteps=%d key digits=%d:", S, R, ksteps, nb);
     for (int v = 0; v < i8_shape_count(); ++v) {
+        if (!i8_variant_ok(v, nb)) continue;
         HIP_OK(hipEventRecord(e0, st));
         if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v)) return -1;
         HIP_OK(hipEventRecord(e1, st));
@@ -80,7 +81,11 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
         HIP_OK(hipEventElapsedTime(&ms, e0, e1));
         int wr, wk;
         i8_shape(v, &wr, &wk);
-        off += snprintf(line + off, sizeof line - off, " %dx%d %.3f ms", wr, wk, ms);
+        const int stg = i8_variant_stages(v);
+        if (stg)
+            off += snprintf(line + off, sizeof line - off, " %dx%d/%d %.3f", wr, wk, stg, ms);
+        else
+            off += snprintf(line + off, sizeof line - off, " %dx%d %.3f", wr, wk, ms);
         if (ms < best_ms) {
             best_ms = ms;
             best = v;
@@ -90,7 +95,8 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
     (void)hipEventDestroy(e1);
     int wr, wk;
     i8_shape(best, &wr, &wk);
-    snprintf(line + off, sizeof line - off, " -> %dx%d\n", wr, wk);
+    snprintf(line + off, sizeof line - off, " ms -> variant %d (%dx%d%s)\n", best, wr, wk,
+             i8_variant_stages(best) ? ", LDS ring" : "");
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = best;
     g_tune_log += line;
@@ -627,6 +633,17 @@ extern "C" int cufe_ctx_model_info(const cufe_ctx *c, int *kb, unsigned int *max
         if (maxabs) maxabs[s] = c->model.maxabs[s];
     }
     return 0;
+}
+
+// Number of int8 contraction variants, and whether variant v runs with nb key
+// digits (CUFE_I8_VARIANT=v forces one).
+extern "C" int cufe_contract_variants(int nb, unsigned int *valid_mask)
+{
+    unsigned int m = 0;
+    for (int v = 0; v < i8_shape_count() && v < 32; ++v)
+        if (i8_variant_ok(v, nb)) m |= 1u << v;
+    if (valid_mask) *valid_mask = m;
+    return i8_shape_count();
 }
 
 // Key digits of the int8 contraction for the loaded model (2 or 3), or 0 when

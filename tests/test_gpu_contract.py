@@ -64,7 +64,10 @@ def test_kernels_agree(gpu, hi, digits):
     lab0, kv0 = predict_with(ctx, R, {})
     ip = (x.astype(np.int64) @ model.sv[:, :l].astype(np.int64).T) % 50241
     assert np.array_equal(kv0.astype(np.int64) % 50241, ip)
-    for env in [{"CUFE_I8_VARIANT": str(v)} for v in range(5)] + [{"CUFE_CONTRACT": "sgpr"},
+    variants = gpu.contract_variants(digits)
+    note(f"variants {variants}")
+    assert len(variants) >= 6
+    for env in [{"CUFE_I8_VARIANT": str(v)} for v in variants] + [{"CUFE_CONTRACT": "sgpr"},
                                                                   {"CUFE_CONTRACT": "lds"}]:
         note(f"kernel {env}")
         lab, kv = predict_with(ctx, R, env)

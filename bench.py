@@ -187,6 +187,48 @@ def serving(cf, ctx, model, x, args, rng):
     return res
 
 
+def single_ops(cf, ctx, rng):
+    """Single-operation kernels at the reference parameter set: forward and
+    inverse NTT of 10,000 polynomials (30,000 limb transforms) on synthetic
+    residues, 10,000 independent one-call Gaussian streams per sampler, one
+    10,000-call S1 stream, and one encrypt / keygen / decrypt at l = 123 through
+    the host-buffer entry points.  The NTT is measured against the butterfly
+    rate of its own butterfly code held in registers (cufe_microbench(7)) and
+    against HBM (32 KiB read + written per limb transform)."""
+    bfly_peak = cf.microbench(7) * 1e9
+    out = {"butterfly_peak_per_s": bfly_peak}
+    for name, inv in (("ntt_forward", False), ("ntt_inverse", True)):
+        ms = cf.time_ntt(10000, inv, 20)
+        limbs = 30000 / (ms * 1e-3)
+        bf = limbs * 24576
+        out[name] = {"limb_transforms_per_s": limbs, "ms_per_10000_polys": ms, "butterflies_per_s": bf,
+                     "frac_of_butterfly_peak": bf / bfly_peak, "hbm_gb_s": limbs * 32768 / 1e9,
+                     "frac_of_hbm": limbs * 32768 / 8e12}
+    for which in (1, 2, 3):
+        ms = cf.time_gauss(10000, which, 2)
+        out[f"gauss_S{which}_10000_streams"] = {"ms": ms, "polys_per_s": 10000 / (ms * 1e-3)}
+    ms = cf.time_gauss_calls(1, 10000, 1, 1)
+    out["gauss_S1_one_stream_10000_calls"] = {"ms": ms, "polys_per_s": 10000 / (ms * 1e-3)}
+    mpk, msk = ctx.get_keys()
+    l = 123
+    mpk_l = np.ascontiguousarray(np.concatenate([mpk[:l], mpk[785:786]]))
+    x = rng.integers(0, 2, (1, l)).astype(np.uint32)
+    y = rng.integers(0, 2, (1, l)).astype(np.uint32)
+    t0 = time.perf_counter()
+    c = cf.encrypt(x, mpk_l, h(b"so2"), h(b"so3"))[0]
+    t1 = time.perf_counter()
+    sk = cf.keygen(y, msk[:l])
+    t2 = time.perf_counter()
+    d = cf.decrypt(c, y, sk)
+    t3 = time.perf_counter()
+    kv = cf.lib().cufe_round_extract2(np.ascontiguousarray(d[0]))
+    out["scheme_l123_host_buffers_ms"] = {"encrypt": 1e3 * (t1 - t0), "keygen": 1e3 * (t2 - t1),
+                                          "decrypt": 1e3 * (t3 - t2),
+                                          "decrypted_equals_inner_product_mod_p":
+                                              int(kv) % P_MOD == int(x[0].astype(np.int64) @ y[0].astype(np.int64))}
+    return out
+
+
 def batched_encrypt(cf, ctx, args, rng):
     """Uneven-length batched encryption: n messages with l_b ~ U{16..785},
     entries U{0..4}, message b on nonce b; one batch call (eager), the same
@@ -253,6 +295,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=4000)
     ap.add_argument("--enc-messages", type=int, default=4096)
     ap.add_argument("--enc-plain", type=int, default=4)
+    ap.add_argument("--no-single-ops", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (known bytes, to scale HBM counters)")
@@ -370,6 +413,9 @@ def main():
                    "predictions_per_s_with_encryption": R / (t_enc + ms_per_step / 1e3)},
         "decryption_check": {"pairs": chk * S, "fraction_equal_mod_p": dec_ok, "zero_decrypted_as_p": dec_p},
     }
+    if not args.no_single_ops:
+        log("single-operation kernels")
+        out["single_ops"] = single_ops(cf, ctx, rng)
     if args.enc_messages > 0:
         out["batched_encrypt"] = batched_encrypt(cf, ctx, args, rng)
     if args.serve_requests > 0:

@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
         "cufe_sife_decrypt": (C.c_int, [_u32p, _u32p, _u32p, _u32p, C.c_int, C.c_int]),
         "cufe_time_ntt": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "cufe_time_gauss": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "cufe_time_gauss_calls": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "cufe_microbench": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
         "cufe_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
         "cufe_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -204,7 +205,8 @@ def time_ntt(npoly: int, inverse: bool, iters: int) -> float:
 
 def microbench(kind: int) -> float:
     """0: fp64 FMA TFLOP/s, 1: int32 mul T ops/s, 2: copy GB/s, 3: AES G blocks/s,
-    4: AES (replicated table) G blocks/s, 5: int8 MFMA T ops/s, 6: fp64 MFMA TFLOP/s."""
+    4: AES (replicated table) G blocks/s, 5: int8 MFMA T ops/s, 6: fp64 MFMA TFLOP/s,
+    7: NTT butterflies in registers G/s."""
     r = C.c_double()
     _check(lib().cufe_microbench(kind, C.byref(r)), "microbench")
     return r.value
@@ -249,6 +251,13 @@ def autotune_report() -> str:
     buf = C.create_string_buffer(n + 1)
     lib().cufe_autotune_report(buf, n + 1)
     return buf.value.decode()
+
+
+def time_gauss_calls(nstreams: int, ncalls: int, which: int, iters: int) -> float:
+    """ms per run of nstreams streams x ncalls calls of sampler `which`."""
+    ms = C.c_float()
+    _check(lib().cufe_time_gauss_calls(nstreams, ncalls, which, iters, C.byref(ms)), "time_gauss_calls")
+    return ms.value
 
 
 def time_gauss(nstreams: int, which: int, iters: int) -> float:

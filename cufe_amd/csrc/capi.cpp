@@ -442,6 +442,34 @@ extern "C" int cufe_time_ntt(int npoly, int inverse, int iters, float *ms_per_la
     return sync_stream(st);
 }
 
+// nstreams streams of ncalls calls each (sampler which), timed per run.
+extern "C" int cufe_time_gauss_calls(int nstreams, int ncalls, int which, int iters, float *ms_per_run)
+{
+    if (nstreams < 1 || nstreams > 65536 || ncalls < 1 || which < 1 || which > 3) {
+        ipfe_set_error("cufe_time_gauss_calls: %d streams x %d calls of S%d", nstreams, ncalls, which);
+        return -1;
+    }
+    Device *dev = device();
+    if (!dev) return -1;
+    hipStream_t st = dev->stream;
+    DevBuf<uint32_t> d;
+    if (d.alloc((size_t)nstreams * ncalls * kPoly)) return -1;
+    std::vector<StreamReq> reqs;
+    for (int t = 0; t < nstreams; ++t) {
+        uint8_t key[32];
+        for (int i = 0; i < 32; ++i) key[i] = (uint8_t)(i * 7 + 3);
+        uint32_t *o = d.get() + (size_t)t * ncalls * kPoly;
+        reqs.push_back(make_stream(key, (uint16_t)t, which, ncalls, o, o + kPoly, (long long)kPoly));
+    }
+    if (run_gauss(reqs, st)) return -1;
+    EventTimer tm(st);
+    for (int it = 0; it < iters; ++it)
+        if (run_gauss(reqs, st)) return -1;
+    float e = tm.stop();
+    if (ms_per_run) *ms_per_run = e / (iters > 0 ? iters : 1);
+    return 0;
+}
+
 extern "C" int cufe_time_gauss(int nstreams, int which, int iters, float *ms_per_launch)
 {
     Device *dev = device();

@@ -5,6 +5,7 @@
 
 #include "../../include/cufe_ipfe.h"
 #include "aes_dev.h"
+#include "ntt_core.h"
 #include "runtime.h"
 
 namespace ipfe {
@@ -124,6 +125,26 @@ __global__ __launch_bounds__(256) void k_mb_mfma_f64(double *out, int iters)
     out[blockIdx.x * blockDim.x + threadIdx.x] = sum[0] + sum[1] + sum[2] + sum[3];
 }
 
+// Register-resident radix-16 CT butterflies (the NTT's butterfly code, twiddle
+// pairs held in registers): the butterfly rate the NTT is measured against.
+__global__ __launch_bounds__(256) void k_mb_bfly(uint32_t *out, int iters, uint32_t w, uint32_t wp, uint32_t q)
+{
+    uint32_t x[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = (threadIdx.x * 2654435761u + 977u * r) % q;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int h = 8; h >= 1; h >>= 1)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if ((r & h) == 0) ct_bfly<false>(x[r], x[r + h], w, wp, q);
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc ^= x[r];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 // Checks that both AES variants produce the same keystream words.
 __global__ void k_mb_aes_check(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global,
                                uint32_t *__restrict__ bad)
@@ -147,7 +168,8 @@ using namespace ipfe;
 // kind 0: fp64 FMA (TFLOP/s), 1: int32 mul_lo+mul_hi+add (T int-ops/s),
 // 2: device copy (GB/s of read+write), 3: AES-256 CTR blocks (G blocks/s),
 // 4: the same with the bank-replicated table (and an equality check),
-// 5: int8 MFMA 32x32x32 (T ops/s), 6: fp64 MFMA 16x16x4 (TFLOP/s).
+// 5: int8 MFMA 32x32x32 (T ops/s), 6: fp64 MFMA 16x16x4 (TFLOP/s),
+// 7: NTT butterflies in registers (G butterflies/s).
 extern "C" int cufe_microbench(int kind, double *result)
 {
     Device *dev = device();
@@ -161,7 +183,22 @@ extern "C" int cufe_microbench(int kind, double *result)
     const int blocks = cus * 8;
     float ms = 0;
     double work = 0;
-    if (kind == 5 || kind == 6) {
+    if (kind == 7) {
+        DevBuf<uint32_t> out;
+        if (out.alloc((size_t)blocks * 256)) return -1;
+        const int iters = 2048;
+        const uint32_t q = qmod(1), w = 1689795u;  // a limb-1 root and its Shoup companion
+        const uint32_t wp = (uint32_t)(((uint64_t)w << 32) / q);
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_OK(hipEventRecord(a, st));
+            hipLaunchKernelGGL(k_mb_bfly, dim3(blocks), dim3(256), 0, st, out.get(), iters, w, wp, q);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipEventRecord(b, st));
+            HIP_OK(hipEventSynchronize(b));
+        }
+        HIP_OK(hipEventElapsedTime(&ms, a, b));
+        work = (double)blocks * 256 * iters * 32;  // butterflies
+    } else if (kind == 5 || kind == 6) {
         DevBuf<double> out;
         if (out.alloc((size_t)blocks * 256)) return -1;
         const int iters = 4096;
@@ -242,6 +279,6 @@ extern "C" int cufe_microbench(int kind, double *result)
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     const double s = ms * 1e-3;
-    *result = (kind == 0 || kind == 1 || kind >= 5) ? work / s / 1e12 : work / s / 1e9;
+    *result = (kind == 0 || kind == 1 || kind == 5 || kind == 6) ? work / s / 1e12 : work / s / 1e9;
     return 0;
 }

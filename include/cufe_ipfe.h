@@ -121,7 +121,7 @@ int cufe_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r);
  * mul_lo + mul_hi + add (T int-ops/s), 2 = device copy (GB/s read + write),
  * 3 = AES-256-CTR keystream (G blocks/s), 4 = the same with the
  * bank-replicated table, 5 = int8 MFMA 32x32x32 (T ops/s), 6 = fp64 MFMA
- * 16x16x4 (TFLOP/s). */
+ * 16x16x4 (TFLOP/s), 7 = NTT butterflies in registers (G/s). */
 int cufe_microbench(int kind, double *result);
 
 /* The scheme at a vector length l <= 785 (uneven-length batching).  With
@@ -142,6 +142,8 @@ int cufe_sife_decrypt(const uint32_t *c, const uint32_t *y, const uint32_t *sk, 
  * returns the mean milliseconds per launch measured with HIP events. */
 int cufe_time_ntt(int npoly, int inverse, int iters, float *ms_per_launch);
 int cufe_time_gauss(int nstreams, int which, int iters, float *ms_per_launch);
+/* nstreams independent streams of ncalls consecutive calls each. */
+int cufe_time_gauss_calls(int nstreams, int ncalls, int which, int iters, float *ms_per_run);
 
 /* ===================================================================== */
 /* 4. Resident-key context and batched prediction                          */

@@ -18,6 +18,9 @@
 // on zero slots and is canonical, so d0 does not depend on which
 // representative the reference's inverse NTT returns for the product.
 #include "crt_dev.h"
+#include <stdlib.h>
+#include <string.h>
+
 #include "kernels_predict.h"
 
 namespace ipfe {
@@ -402,6 +405,69 @@ __global__ void k_decide(const uint32_t *__restrict__ kval, int R, int S, SvmMod
 }
 
 // ---------------------------------------------------------------------------
+// The same decision with one wave per row.  The products are independent of
+// the order, so the 64 lanes form them in parallel; only the sum is ordered.
+// EXACT_INT (integer coefficients with sum |coef| * p < 2^53): every product
+// and partial sum is an exact integer, so any order gives the same bits and
+// the lanes reduce in a tree.  Otherwise the lanes stage the rounded products
+// of 512 terms at a time in LDS and lane 0 adds them in the reference order.
+// ---------------------------------------------------------------------------
+template <bool EXACT_INT>
+__global__ __launch_bounds__(256) void k_decide_wave(const uint32_t *__restrict__ kval, int R, int S, SvmModelDev m,
+                                                     double *__restrict__ labels)
+{
+    constexpr int CH = 512;
+    __shared__ double prod[4][EXACT_INT ? 1 : CH];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + wv;
+    if (r >= R) return;  // whole wave; no block barriers below
+    const uint32_t *kv = kval + (size_t)r * S;
+    int vote[SVM_MAX_CLASS];
+    for (int i = 0; i < m.nr_class; ++i) vote[i] = 0;
+    int p = 0;
+    for (int i = 0; i < m.nr_class; ++i)
+        for (int j = i + 1; j < m.nr_class; ++j) {
+            const double *coef1 = m.sv_coef + (size_t)(j - 1) * S;
+            const double *coef2 = m.sv_coef + (size_t)i * S;
+            const int n1 = m.nSV[i], total = m.nSV[i] + m.nSV[j];
+            // term t: SV start[i] + t (coef1) for t < n1, else start[j] + t - n1 (coef2)
+            auto term = [&](int t) {
+                const int sv = t < n1 ? m.start[i] + t : m.start[j] + (t - n1);
+                const double c = t < n1 ? coef1[sv] : coef2[sv];
+                return __dmul_rn(c, (double)kv[sv]);
+            };
+            double sum = 0.0;
+            if (EXACT_INT) {
+                double part = 0.0;
+                for (int t = lane; t < total; t += 64) part += term(t);
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+                sum = part;
+            } else {
+                for (int b0 = 0; b0 < total; b0 += CH) {
+                    const int cnt = min(CH, total - b0);
+                    for (int t = lane; t < cnt; t += 64) prod[wv][t] = term(b0 + t);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) {
+#pragma unroll 16
+                        for (int k = 0; k < cnt; ++k) sum = __dadd_rn(sum, prod[wv][k]);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            sum = __dsub_rn(sum, m.rho[p]);
+            if (sum > 0) ++vote[i]; else ++vote[j];
+            ++p;
+        }
+    if (lane == 0) {
+        int best = 0;
+        for (int i = 1; i < m.nr_class; ++i)
+            if (vote[i] > vote[best]) best = i;
+        labels[r] = (double)m.label[best];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Compact encryption: coefficient 0 of c_i for i < l and every row,
 //   c_i[0] = add_mod( N^-1 sum_k pk_i[k] r_hat[k], f_i[0] ) + floor(Q/p) x_i  (mod q)
 // (coefficient 0 of GS_reverse is N^-1 times the sum of its inputs).  One block
@@ -548,7 +614,13 @@ int launch_crt_kvalue(const uint32_t *D, size_t n, uint32_t *kval, hipStream_t s
 int launch_decide(const uint32_t *kval, int R, int S, const SvmModelDev &m, double *labels, hipStream_t st)
 {
     if (R <= 0) return 0;
-    hipLaunchKernelGGL(k_decide, dim3((R + 127) / 128), dim3(128), 0, st, kval, R, S, m, labels);
+    const char *e = getenv("CUFE_DECIDE");
+    if (e && strcmp(e, "thread") == 0)
+        hipLaunchKernelGGL(k_decide, dim3((R + 127) / 128), dim3(128), 0, st, kval, R, S, m, labels);
+    else if (m.exact_int)
+        hipLaunchKernelGGL(k_decide_wave<true>, dim3((R + 3) / 4), dim3(256), 0, st, kval, R, S, m, labels);
+    else
+        hipLaunchKernelGGL(k_decide_wave<false>, dim3((R + 3) / 4), dim3(256), 0, st, kval, R, S, m, labels);
     HIP_OK(hipGetLastError());
     return 0;
 }

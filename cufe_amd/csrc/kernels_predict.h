@@ -14,6 +14,7 @@ struct SvmModelDev {
     int label[SVM_MAX_CLASS];
     const double *sv_coef;  // [nr_class-1][S]
     const double *rho;      // [nr_class*(nr_class-1)/2]
+    int exact_int;          // integer coefficients, sum |coef| * p < 2^53: sums exact in any order
 };
 
 int launch_contract_keys(const uint32_t *sk, int S, int K, int sign, const SparseY &y, double *kd, unsigned int *maxabs,

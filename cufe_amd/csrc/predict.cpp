@@ -186,6 +186,18 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
     }
     md.sv_coef = coef.get();
     md.rho = rho.get();
+    // decision sums are exact (any order gives the same bits) when every
+    // coefficient is an integer and sum |coef| * p (p bounds a key value) < 2^53
+    {
+        bool integral = true;
+        double abs_sum = 0.0;
+        for (int k = 0; k < (m->nr_class - 1) * nS; ++k) {
+            const double c = m->sv_coef[k];
+            integral = integral && c == floor(c);
+            abs_sum += fabs(c);
+        }
+        md.exact_int = integral && abs_sum * 50241.0 < 9007199254740992.0;
+    }
     mdev = md;
     // reduction interval (16-product steps) of the fp64 contraction: every
     // partial sum stays an exact integer below 2^53

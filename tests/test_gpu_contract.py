@@ -107,3 +107,41 @@ def test_large_keys_fall_back_to_fp64(gpu, po):
             d = po.decrypt(c_pad, y_pad, sk)
             assert int(kv[r, v]) == po.crt_round(int(d[0, 0]), int(d[1, 0]), int(d[2, 0]))
     ctx.close()
+
+
+@pytest.mark.parametrize("nr_class,integer_coef", [(2, True), (2, False), (4, False), (3, True)])
+def test_decision_kernels_agree(gpu, nr_class, integer_coef):
+    """The wave-per-row vote (exact-integer tree sums, or lane-0 ordered sums of
+    rounded products) gives the labels of the one-thread-per-row vote that
+    follows main.c:338-389 literally."""
+    rng = np.random.default_rng(40 + nr_class)
+    l, R, S = 64, 300, 700
+    ctx = gpu.Context()
+    ctx.setup(h(b"dA"), h(b"dS"), h(b"dE"), 785)
+    x = rng.integers(0, 5, (R, l)).astype(np.uint32)
+    ctx.rows_reserve(R, l)
+    ctx.encrypt_rows(x, h(b"d2"), h(b"d3"), nonce0=0)
+    sv = np.zeros((S, 785), np.uint32)
+    for v in range(S):
+        sv[v, rng.choice(np.arange(1, l), 6, replace=False)] = rng.integers(1, 4, 6)
+    base = S // nr_class
+    nSV = [base] * (nr_class - 1) + [S - base * (nr_class - 1)]
+    npair = nr_class * (nr_class - 1) // 2
+    if integer_coef:
+        coef = rng.integers(-9, 10, (nr_class - 1, S)).astype(np.float64)
+        rho = rng.integers(-50, 50, npair) + 0.5
+    else:
+        coef = rng.normal(0, 1, (nr_class - 1, S))
+        rho = rng.normal(0, 30, npair)
+    ctx.set_model(gpu.SvmModel(nr_class, nSV, list(range(1, nr_class + 1)), coef, rho, sv))
+    lab, kv = predict_with(ctx, R, {})
+    lab_t, kv_t = predict_with(ctx, R, {"CUFE_DECIDE": "thread"})
+    assert np.array_equal(kv, kv_t)
+    assert np.array_equal(lab, lab_t)
+    from cufe_amd import svm_io
+
+    model = gpu.SvmModel(nr_class, nSV, list(range(1, nr_class + 1)), coef, rho, sv)
+    for r in range(0, R, 37):
+        assert lab[r] == svm_io.decide(model, kv[r].astype(np.float64)), r
+    note(f"decision nr_class={nr_class} integer={integer_coef}: labels {np.unique(lab, return_counts=True)}")
+    ctx.close()

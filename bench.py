@@ -84,6 +84,54 @@ def cpu_baseline(cf, ctx, model, x_row, seconds: float):
                       f"extrapolated to {S} SVs"}
 
 
+def _cpu_worker(args):
+    """One spawned process: times the oracle's svm_predict_cpu restatement on
+    support vectors [v0, v0 + k) repeatedly for `seconds`; returns pairs done.
+    It loads only the CPU oracle (never the HIP runtime)."""
+    path, v0, k, seconds = args
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    d = np.load(path)
+    c, msk, sv = d["c"], d["msk"], np.ascontiguousarray(d["sv"][v0:v0 + k])
+    L = msk.shape[0]
+    kvk = np.empty(k)
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        po.orc().orc_svm_predict_cpu(c, L, msk, 2, k, np.array([k, 0], np.int32), np.array([1, -1], np.int32),
+                                     np.ascontiguousarray(np.ones(k)), np.array([0.0]), sv, kvk, 0)
+        done += k
+    return done, time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(cf, ctx, model, x_row, seconds: float):
+    """The same CPU restatement on every host core the box grants (at most 16),
+    one spawned process per core, each on its own slice of the support vectors."""
+    import multiprocessing as mp
+    import tempfile
+
+    mpk, msk = ctx.get_keys()
+    L = msk.shape[0]
+    xf = np.zeros((1, L), np.uint32)
+    xf[0, : x_row.size] = x_row
+    c = cf.encrypt(xf, mpk, h(b"cpu2"), h(b"cpu3"))[0]
+    cores = max(1, min(16, os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "cpu_inputs.npz")
+        np.savez(path, c=c, msk=msk, sv=model.sv)
+        k = 2
+        jobs = [(path, (i * k) % (model.l - k), k, seconds) for i in range(cores)]
+        with mp.get_context("spawn").Pool(cores) as pool:
+            res = pool.map(_cpu_worker, jobs)
+    pairs = sum(r[0] for r in res)
+    wall = max(r[1] for r in res)
+    rate_pairs = pairs / wall
+    return {"value": rate_pairs / model.l, "unit": "predictions/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} processes x {seconds:.0f} s, {pairs} (row, SV) pairs of svm_predict_cpu "
+                      f"({rate_pairs:.1f} pairs/s), extrapolated to {model.l} SVs per prediction"}
+
+
 def plain_gpu(cf, ctx, model, x, nrows: int):
     """The plain per-call form: rlwe_sife_encrypt_gpu per row is not timed; per row
     svm_predict_gpu = keygen_gpu + decrypt_gmp_gpu + host round_extract2 + vote."""
@@ -429,6 +477,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 core)")
         out["cpu_baseline"] = cpu_baseline(cf, ctx, model, x[0], args.cpu_seconds)
+        log("cpu baseline (oracle, all granted cores)")
+        out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(cf, ctx, model, x[0], args.cpu_seconds / 2)
     ctx.close()
     print(json.dumps(out))
 

@@ -65,6 +65,10 @@ def lib() -> C.CDLL:
         "cufe_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
         "cufe_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
         "cufe_contract_variants": (C.c_int, [C.c_int, C.POINTER(C.c_uint)]),
+        "cufe_wire_size": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+        "cufe_wire_pack": (C.c_int, [C.c_int, _u32p, C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_void_p,
+                                     C.c_size_t]),
+        "cufe_wire_parse": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p]),
         "cufe_host_free": (None, [C.c_void_p]),
         "rlwe_sife_setup_gpu": (None, [_u32p, _u32p, _u8p, _u8p, C.POINTER(C.c_float)]),
         "rlwe_sife_encrypt_gpu": (None, [_u32p, _u32p, _u32p, _u8p, _u8p, C.c_int, C.POINTER(C.c_float)]),
@@ -236,6 +240,63 @@ def host_empty(shape, dtype=np.uint32) -> np.ndarray:
     cbuf = (C.c_uint8 * (count * dt.itemsize)).from_address(owner.ptr)
     cbuf._owner = owner
     return np.frombuffer(cbuf, dtype=dt, count=count).reshape(shape)
+
+
+# ---- wire format (requests, ciphertexts, keys) ------------------------------------
+WIRE_ROWS, WIRE_CIPHERTEXTS, WIRE_PUBLIC_KEY, WIRE_FUNCTIONAL_KEYS = 1, 2, 3, 4
+
+
+class _WireInfo(C.Structure):
+    _fields_ = [("kind", C.c_int), ("count", C.c_int), ("l", C.c_int), ("tag", C.c_uint32),
+                ("data", C.c_void_p), ("c0", C.c_void_p)]
+
+
+def wire_pack(kind: int, data, data2=None, tag: int = 0) -> np.ndarray:
+    """Blob (uint8 array) of one kind: rows (data = c_L [n][3][N], data2 = c0
+    [n][l][3]), ciphertexts [n][l+1][3][N], public key [l+1][3][N] or
+    functional keys [n][3][N]."""
+    data = _u32(data)
+    if kind == WIRE_ROWS:
+        data2 = _u32(data2)
+        count, l = data2.shape[0], data2.shape[1]
+        assert data.shape == (count, NMODULI, N) and data2.shape == (count, l, NMODULI)
+    elif kind == WIRE_CIPHERTEXTS:
+        count, l = data.shape[0], data.shape[1] - 1
+    elif kind == WIRE_PUBLIC_KEY:
+        count, l = 1, data.shape[0] - 1
+    else:
+        count, l = data.shape[0], 0
+    size = lib().cufe_wire_size(kind, count, l)
+    if size == 0:
+        raise CufeError(f"wire_pack: invalid shape for kind {kind}")
+    out = np.empty(size, np.uint8)
+    _check(lib().cufe_wire_pack(kind, data, data2.ctypes.data if data2 is not None else None, count, l, tag,
+                                out.ctypes.data, size), "wire_pack")
+    return out
+
+
+def wire_parse(blob) -> dict:
+    """Validate a blob and return its fields; arrays are views into the blob
+    (copied first into an aligned uint8 array if it is not one)."""
+    b = np.frombuffer(blob, np.uint8) if isinstance(blob, (bytes, bytearray)) else np.asarray(blob, np.uint8)
+    if b.ctypes.data % 4:
+        b = b.copy()
+    info = _WireInfo()
+    _check(lib().cufe_wire_parse(b.ctypes.data, b.size, C.byref(info)), "wire_parse")
+    base = b.ctypes.data
+    words = b[info.data - base:].view(np.uint32)
+    k, n, l = info.kind, info.count, info.l
+    out = {"kind": k, "count": n, "l": l, "tag": info.tag, "blob": b}
+    if k == WIRE_ROWS:
+        out["cl"] = words[: n * NMODULI * N].reshape(n, NMODULI, N)
+        out["c0"] = b[info.c0 - base:].view(np.uint32)[: n * l * NMODULI].reshape(n, l, NMODULI)
+    elif k == WIRE_CIPHERTEXTS:
+        out["data"] = words[: n * (l + 1) * NMODULI * N].reshape(n, l + 1, NMODULI, N)
+    elif k == WIRE_PUBLIC_KEY:
+        out["data"] = words[: (l + 1) * NMODULI * N].reshape(l + 1, NMODULI, N)
+    else:
+        out["data"] = words[: n * NMODULI * N].reshape(n, NMODULI, N)
+    return out
 
 
 def contract_variants(nb: int) -> list:

@@ -270,6 +270,38 @@ void cufe_server_destroy(cufe_server *srv);
 int cufe_host_alloc(size_t bytes, void **out);
 void cufe_host_free(void *p);
 
+/* ===================================================================== */
+/* 6. Wire format for requests, ciphertexts and keys                       */
+/*    No counterpart in the reference, which passes in-memory arrays only.  */
+/*    A 64-byte little-endian header (magic "CUFEIPFE", version 1, kind,   */
+/*    limbs, N, count, slots l, the three moduli, a caller tag, CRC-32 of   */
+/*    the payload, payload bytes) followed by the residue arrays exactly as */
+/*    the API takes them; every residue lies in [0, q] of its limb.         */
+/* ===================================================================== */
+
+enum {
+    CUFE_WIRE_ROWS = 1,            /* c_L [count][3][4096], then c0 [count][l][3] */
+    CUFE_WIRE_CIPHERTEXTS = 2,     /* [count][l+1][3][4096], index l = c_L */
+    CUFE_WIRE_PUBLIC_KEY = 3,      /* [l+1][3][4096], index l = a_hat, count 1 */
+    CUFE_WIRE_FUNCTIONAL_KEYS = 4  /* [count][3][4096] */
+};
+
+typedef struct {
+    int kind, count, l;
+    uint32_t tag;
+    const uint32_t *data; /* payload inside the blob */
+    const uint32_t *c0;   /* rows: the c0 block inside the blob; else NULL */
+} cufe_wire_info;
+
+/* Bytes of a blob of this kind and shape (0 if the shape is invalid). */
+size_t cufe_wire_size(int kind, int count, int l);
+/* Packs data (and, for rows, data2 = c0) into out. */
+int cufe_wire_pack(int kind, const uint32_t *data, const uint32_t *data2, int count, int l, uint32_t tag, void *out,
+                   size_t cap);
+/* Validates a blob (header, parameter set, length, CRC, residue ranges) and
+ * describes it; the pointers point into the blob (4-byte aligned blobs). */
+int cufe_wire_parse(const void *blob, size_t len, cufe_wire_info *info);
+
 #ifdef __cplusplus
 }
 #endif

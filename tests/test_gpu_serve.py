@@ -167,6 +167,24 @@ def test_pinned_requests(gpu, env):
         srv.close()
 
 
+def test_wire_requests(gpu, env):
+    """Requests packed in the wire format and parsed back are served from the
+    blob's own memory with the same results."""
+    srv = start_server(gpu, env, max_batch_rows=256)
+    try:
+        reqs = []
+        for mid, r0, n in ((0, 3, 20), (1, 40, 33), (3, 0, 24)):
+            cl, c0 = request(env, mid, r0, n)
+            p = gpu.wire_parse(gpu.wire_pack(gpu.WIRE_ROWS, cl, c0, tag=mid))
+            assert p["tag"] == mid and np.array_equal(p["cl"], cl)
+            reqs.append((srv.submit(mid, p["cl"], p["c0"], want_kvalues=True), mid, r0, n, p))
+        for t, mid, r0, n, _ in reqs:
+            lab, kv, _ = srv.wait(t, 60000)
+            check(env, mid, r0, n, lab, kv)
+    finally:
+        srv.close()
+
+
 def test_poisson_replay(gpu, env):
     srv = start_server(gpu, env, max_batch_rows=1024)
     try:

@@ -427,13 +427,20 @@ def main():
         roof = {"bound": "valu-fp64", "kernel": "k_contract_sgpr", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "work_per_launch": f"{R} rows x {S} keys x {K} columns (1 fp64 FMA each) for one limb"}
     achieved = ops_per_launch / (avg_launch_ms * 1e-3) / 1e12
+    # HBM traffic per launch of the variant the autotuner chose for this shape,
+    # from the calibrated FETCH_SIZE / WRITE_SIZE passes committed in profiles/
     traffic = None
+    variant = None
+    for line in cf.autotune_report().splitlines():
+        if line.startswith(f"S={S} R={R} ") and "-> variant " in line:
+            variant = line.split("-> variant ")[1].split()[0]
     tfile = os.path.join(ROOT, "profiles", "k_contract_i8_traffic.json")
-    if i8 and os.path.exists(tfile):
+    if i8 and variant is not None and os.path.exists(tfile):
         with open(tfile) as f:
             t = json.load(f)
-        if t.get("rows") == R and t.get("svs") == S:
-            traffic = t["hbm_bytes_per_launch"]
+        if t.get("rows") == R and t.get("svs") == S and variant in t.get("variants", {}):
+            traffic = t["variants"][variant]["hbm_bytes_per_launch"]
+            roof["kernel"] = t["variants"][variant]["kernel"]
     roof.update({"achieved": achieved, "frac": achieved / roof["peak"], "traffic": traffic,
                  "avg_launch_ms": avg_launch_ms,
                  "modular_macs_per_s": 3.0 * R * S * K * args.steps / (contract_ms * 1e-3)})

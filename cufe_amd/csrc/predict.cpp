@@ -67,6 +67,7 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
     HIP_OK(hipEventCreate(&e1));
     // one untimed launch loads the code objects
     if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, 0)) return -1;
+    std::vector<float> times(i8_shape_count(), -1.0f);
     int best = 0;
     float best_ms = 1e30f;
     char line[512];
@@ -86,10 +87,20 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
             off += snprintf(line + off, sizeof line - off, " %dx%d/%d %.3f", wr, wk, stg, ms);
         else
             off += snprintf(line + off, sizeof line - off, " %dx%d %.3f", wr, wk, ms);
+        times[v] = ms;
         if (ms < best_ms) {
             best_ms = ms;
             best = v;
         }
+    }
+    // Ties within 2% go to the variant with the most rows per block tile: it
+    // re-reads the key operand from beyond L2 the fewest times.
+    for (int v = 0; v < i8_shape_count(); ++v) {
+        if (times[v] < 0 || times[v] > 1.02f * best_ms) continue;
+        int wr, wk, bwr, bwk;
+        i8_shape(v, &wr, &wk);
+        i8_shape(best, &bwr, &bwk);
+        if (wr > bwr || (wr == bwr && times[v] < times[best])) best = v;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -259,12 +259,12 @@ int RowStore::reserve(int cap_rows, int len)
     const int ld = IPFE_N + ((len + 15) / 16) * 16;
     const int cp = ((cap_rows + 63) / 64) * 64;  // whole 64-row tiles of the blocked store
     const int ks = (IPFE_N + ((len + 31) / 32) * 32) / 32;
-    if (A.alloc((size_t)3 * cp * ld) || c0.alloc((size_t)cp * len * 3) || A64.alloc((size_t)3 * (cp + 64) * ld) ||
-        Adig.alloc(3 * rows_digit_limb_bytes(cp, ks)))
+    A64.reset();
+    if (A.alloc((size_t)3 * cp * ld) || c0.alloc((size_t)cp * len * 3) || Adig.alloc(3 * rows_digit_limb_bytes(cp, ks)))
         return -1;
     HIP_OK(hipMemset(A.get(), 0, (size_t)3 * cp * ld * 4));
-    HIP_OK(hipMemset(A64.get(), 0, (size_t)3 * (cp + 64) * ld * 8));
     HIP_OK(hipMemset(Adig.get(), 0, Adig.n));
+    a64_ok.assign((size_t)cp / 64, 0);
     cap = cp;
     l = len;
     ldr = ld;
@@ -284,8 +284,28 @@ int RowStore::put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev,
     if (launch_rows_c0(c0_dev, n, src_l, A.get(), cap, r0, ldr, st)) return -1;
     HIP_OK(hipMemcpy2DAsync(c0.get() + (size_t)r0 * l * 3, (size_t)l * 12, c0_dev, (size_t)src_l * 12,
                             (size_t)src_l * 12, n, hipMemcpyDeviceToDevice, st));
-    if (launch_rows_blocked(A.get(), cap, cap + 64, ldr, r0, n, A64.get(), st)) return -1;
     if (launch_rows_digits(A.get(), cap, ldr, r0, n, ksteps, Adig.get(), st)) return -1;
+    for (int b = r0 / 64; b <= (r0 + n - 1) / 64; ++b) a64_ok[b] = 0;
+    return 0;
+}
+
+int RowStore::ensure_a64(int r0, int R, hipStream_t st) const
+{
+    if (!A64.get()) {
+        if (A64.alloc((size_t)3 * (cap + 64) * ldr)) return -1;
+        if (launch_fill_i32((int *)A64.get(), 0, A64.n * 2, st)) return -1;
+    }
+    const int b1 = (r0 + R + 63) / 64;
+    for (int b = r0 / 64; b < b1;) {
+        if (a64_ok[b]) {
+            ++b;
+            continue;
+        }
+        int e = b;
+        while (e < b1 && !a64_ok[e]) a64_ok[e++] = 1;
+        if (launch_rows_blocked(A.get(), cap, cap + 64, ldr, b * 64, (e - b) * 64, A64.get(), st)) return -1;
+        b = e;
+    }
     return 0;
 }
 
@@ -336,7 +356,8 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
                 rc = launch_contract_slow(As, Bs, R, S, m.kkey, rs.ldr, m.kkey, s, dev->barrett, Ds, st);
             else if (mode >= 1 && r0 % 16 == 0)
                 // kb counts 16-product steps; this kernel's steps are 8 products
-                rc = launch_contract_sgpr(rs.A64.get() + ((size_t)s * (rs.cap + 64) + r0) * rs.ldr, Bs, R, S, m.kkey,
+                rc = rs.ensure_a64(r0, R, st) ||
+                     launch_contract_sgpr(rs.A64.get() + ((size_t)s * (rs.cap + 64) + r0) * rs.ldr, Bs, R, S, m.kkey,
                                           rs.ldr, m.kkey, s, 2 * m.kb[s], Ds, st);
             else
                 rc = launch_contract(As, Bs, R, S, m.kkey, rs.ldr, m.kkey, s, m.kb[s], Ds, st);

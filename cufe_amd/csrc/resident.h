@@ -5,6 +5,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/cufe_ipfe.h"
 #include "kernels_predict.h"
 #include "runtime.h"
@@ -46,11 +48,16 @@ struct Model {
 struct RowStore {
     int l = 0, cap = 0, ldr = 0, ksteps = 0;
     DevBuf<int32_t> A;
-    DevBuf<double> A64;
     DevBuf<int8_t> Adig;
     DevBuf<uint32_t> c0;
+    // fp64 blocked copy, built on demand (only the fp64 fallback kernel reads
+    // it): allocated at first use, converted per 64-row block when stale
+    mutable DevBuf<double> A64;
+    mutable std::vector<uint8_t> a64_ok;
 
     int reserve(int cap, int l);
+    // rows [r0, r0+R) of the fp64 blocked copy current
+    int ensure_a64(int r0, int R, hipStream_t st) const;
     // rows [r0, r0+n) from device buffers: cl [n][3][N] canonical, c0
     // [n][src_l][3] with src_l <= l slots (default l); slots past src_l read 0
     int put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev, hipStream_t st, int src_l = -1);

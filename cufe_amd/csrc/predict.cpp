@@ -335,6 +335,21 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
     Device *dev = device();
     if (!dev) return -1;
     const int S = m.S;
+    if (mode == 2 && m.i8 && row0 % 32 != 0 && nrows > 0) {
+        // The int8 kernel reads rows in 32-row blocks: predict from the block
+        // start and keep the requested rows (results are per row, so the extra
+        // rows change nothing).
+        const int lead = row0 % 32;
+        std::vector<double> lab((size_t)nrows + lead);
+        std::vector<uint32_t> kv(kval_host ? ((size_t)nrows + lead) * S : 0);
+        if (predict_rows(m, rs, row0 - lead, nrows + lead, sc, scratch_rows, mode, lab.data(),
+                         kval_host ? kv.data() : nullptr, tm, st))
+            return -1;
+        HIP_OK(hipStreamSynchronize(st));
+        if (labels_host) memcpy(labels_host, lab.data() + lead, sizeof(double) * nrows);
+        if (kval_host) memcpy(kval_host, kv.data() + (size_t)lead * S, sizeof(uint32_t) * nrows * S);
+        return 0;
+    }
     const int chunk = std::max(64, scratch_rows / 64 * 64);
     if (sc.reserve(std::min(chunk, ((nrows + 63) / 64) * 64), S)) return -1;
     if (tm) tm->ms[0] = tm->ms[1] = tm->ms[2] = 0;

@@ -111,6 +111,13 @@ def test_predict_row_subsets_and_chunking(gpu, keyed_ctx):
     lab_all, kv_all, _ = ctx.predict(0, R, want_kvalues=True)
     lab_part, kv_part, _ = ctx.predict(77, 101, want_kvalues=True)
     assert np.array_equal(lab_all[77:178], lab_part) and np.array_equal(kv_all[77:178], kv_part)
+    # an unaligned range on the fp64 LDS kernel gives the same values
+    os.environ["CUFE_CONTRACT"] = "lds"
+    try:
+        lab_lds, kv_lds, _ = ctx.predict(33, 200, want_kvalues=True)
+    finally:
+        os.environ.pop("CUFE_CONTRACT", None)
+    assert np.array_equal(kv_all[33:233], kv_lds) and np.array_equal(lab_all[33:233], lab_lds)
     # decryption recovers <x, y> (mod p, with p standing for 0)
     ip = (x.astype(np.int64) @ model.sv[:, :l].astype(np.int64).T) % 50241
     assert np.array_equal(kv_all.astype(np.int64) % 50241, ip)

@@ -393,10 +393,12 @@ def main():
     # predict() synchronises the device before it returns, so the timed region
     # starts and ends with an idle GPU
     contract_ms = 0.0
+    limb_ms = [0.0, 0.0, 0.0]
     t0 = time.perf_counter()
     for _ in range(args.steps):
         labels, _, ms = ctx.predict(0, R)
         contract_ms += ms[0]
+        limb_ms = [a + b for a, b in zip(limb_ms, ctx.contract_limb_ms())]
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = R * args.steps / elapsed
@@ -411,39 +413,47 @@ def main():
 
     chunk_rows = max(64, (1 << 30) // S)
     chunks = (R + chunk_rows - 1) // chunk_rows
-    launches = 3 * chunks * args.steps
-    avg_launch_ms = contract_ms / launches
     K = 4096 + (max(int(np.nonzero(sv.any(axis=0))[0].max()) + 1, 1) + 15) // 16 * 16  # key columns
     nb = ctx.contract_digits()
     i8 = nb > 0 and os.environ.get("CUFE_CONTRACT", "") not in ("sgpr", "lds")
-    if i8:
-        # 4 x nb signed-digit int8 GEMMs per limb, one MAC = 2 ops
-        ops_per_launch = 2.0 * 4 * nb * R * S * K / chunks
-        roof = {"bound": "mfma", "kernel": "k_contract_i8", "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                "work_per_launch": f"{R} rows x {S} keys x {K} columns x {4 * nb} digit products (int8 MFMA) "
-                                   f"for one limb"}
-    else:
-        ops_per_launch = 2.0 * R * S * K / chunks  # one fp64 FMA per modular MAC
-        roof = {"bound": "valu-fp64", "kernel": "k_contract_sgpr", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "work_per_launch": f"{R} rows x {S} keys x {K} columns (1 fp64 FMA each) for one limb"}
-    achieved = ops_per_launch / (avg_launch_ms * 1e-3) / 1e12
-    # HBM traffic per launch of the variant the autotuner chose for this shape,
-    # from the calibrated FETCH_SIZE / WRITE_SIZE passes committed in profiles/
-    traffic = None
-    variant = None
+    # The dominant kernel: the contraction of limbs 1 and 2 (two of the three
+    # launches per chunk; limb 0, whose residues fit three int8 digits, runs a
+    # kernel with 3/4 of the MFMA work and is reported beside it).
+    tuned = {}
     for line in cf.autotune_report().splitlines():
         if line.startswith(f"S={S} R={R} ") and "-> variant " in line:
-            variant = line.split("-> variant ")[1].split()[0]
+            tuned["limb 0" if " limb 0:" in line else "limbs 1-2"] = line.split("-> variant ")[1].split("(")[1].rstrip(")")
     tfile = os.path.join(ROOT, "profiles", "k_contract_i8_traffic.json")
-    if i8 and variant is not None and os.path.exists(tfile):
+    tdata = {}
+    if os.path.exists(tfile):
         with open(tfile) as f:
             t = json.load(f)
-        if t.get("rows") == R and t.get("svs") == S and variant in t.get("variants", {}):
-            traffic = t["variants"][variant]["hbm_bytes_per_launch"]
-            roof["kernel"] = t["variants"][variant]["kernel"]
-    roof.update({"achieved": achieved, "frac": achieved / roof["peak"], "traffic": traffic,
-                 "avg_launch_ms": avg_launch_ms,
-                 "modular_macs_per_s": 3.0 * R * S * K * args.steps / (contract_ms * 1e-3)})
+        if t.get("rows") == R and t.get("svs") == S:
+            tdata = t.get("variants", {})
+
+    def roofline(limbs, digits_a):
+        launches = len(limbs) * chunks * args.steps
+        avg = sum(limb_ms[s] for s in limbs) / launches
+        if i8:
+            ops = 2.0 * digits_a * nb * R * S * K / chunks  # digits_a x nb int8 GEMMs, one MAC = 2 ops
+            label = tuned.get("limb 0" if limbs == [0] else "limbs 1-2")
+            r = {"bound": "mfma", "kernel": tdata.get(label, {}).get("kernel", "k_contract_i8"), "variant": label,
+                 "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                 "work_per_launch": f"{R} rows x {S} keys x {K} columns x {digits_a * nb} digit products "
+                                    f"(int8 MFMA) for one limb"}
+            traffic = tdata.get(label, {}).get("hbm_bytes_per_launch")
+        else:
+            ops = 2.0 * R * S * K / chunks  # one fp64 FMA per modular MAC
+            r = {"bound": "valu-fp64", "kernel": "k_contract_sgpr", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "work_per_launch": f"{R} rows x {S} keys x {K} columns (1 fp64 FMA each) for one limb"}
+            traffic = None
+        achieved = ops / (avg * 1e-3) / 1e12
+        r.update({"achieved": achieved, "frac": achieved / r["peak"], "traffic": traffic, "avg_launch_ms": avg})
+        return r
+
+    roof = roofline([1, 2], 4)
+    roof["limb0"] = roofline([0], 3 if i8 and tuned.get("limb 0", "").endswith("i3") else 4)
+    roof["modular_macs_per_s"] = 3.0 * R * S * K * args.steps / (contract_ms * 1e-3)
     out = {
         "metric": "IPFE-SVM predictions/sec on 1 MI355X, batched vs plain (bit-exact); p99 ms",
         "value": value,

@@ -375,6 +375,7 @@ def _ctx_sigs(lb):
         "cufe_ctx_get_row": (C.c_int, [p, C.c_int, _u32p, _u32p]),
         "cufe_ctx_export_rows": (C.c_int, [p, C.c_int, C.c_int, _u32p, _u32p]),
         "cufe_ctx_contract_digits": (C.c_int, [p]),
+        "cufe_ctx_contract_limb_ms": (C.c_int, [p, C.POINTER(C.c_float)]),
         "cufe_svm_predict_gpu": (C.c_double, [C.POINTER(_SvmModelC), _u32p, _u32p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
@@ -452,6 +453,12 @@ class Context:
         _check(lib().cufe_ctx_predict(self._h, row0, nrows, labels, kv.ctypes.data if kv is not None else None, ms),
                "ctx_predict")
         return labels, kv, list(ms)
+
+    def contract_limb_ms(self):
+        """Contraction time per limb (ms) of the last predict call."""
+        ms = (C.c_float * 3)()
+        _check(lib().cufe_ctx_contract_limb_ms(self._h, ms), "ctx_contract_limb_ms")
+        return list(ms)
 
     def model_info(self):
         kb = (C.c_int * 3)()

@@ -20,6 +20,10 @@
 // lane l holds row / key (l mod 32) and k = 16 (l / 32) + j, j < 16 -- the
 // operand layout of v_mfma_i32_32x32x32_i8, whose result lane l, register i
 // is D[8 (i / 4) + 4 (l / 32) + i mod 4][l mod 32].
+#include <stdio.h>
+
+#include <type_traits>
+
 #include "kernels_predict.h"
 
 namespace ipfe {
@@ -281,6 +285,162 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 }
 
 // ---------------------------------------------------------------------------
+// Pipelined variant with the step's work interleaved by hand.  Of a step's
+// DA x NB MFMAs, the first two are issued before the wait for the next ring
+// slot and the barrier; the third is followed by the ring loads of step
+// ks + STAGES, the fourth by the LDS reads of the next step's key fragments,
+// and each later one by the read of one next row fragment.  Scheduling
+// barriers keep that order, so the barrier and the LDS latency fall under
+// queued MFMAs instead of between steps, and a second register set holds the
+// next step's fragments.
+//
+// DA = 3 uses only row digits 0..2: valid when every row value lies in
+// [-8421504, 8355711] (digit 3 is then zero), which the row digit store
+// guarantees for limb 0 (q < 2^24).  The zero digit plane is still loaded, so
+// every wave issues the same number of ring loads.
+//
+// The main loop runs while the ring still has steps to load; the last
+// STAGES + 1 steps (at most) wait with the conservative vmcnt(0).
+// ---------------------------------------------------------------------------
+template <int WR, int WK, int NB, int STAGES, int DA>
+__global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *__restrict__ A,
+                                                                    const v4i *__restrict__ B, int R, int S,
+                                                                    int ksteps, long long a_stride,
+                                                                    long long b_stride, uint32_t q,
+                                                                    uint32_t *__restrict__ D, int ntk, int ntiles)
+{
+    constexpr int W = WR * WK;
+    constexpr int F = WR * DG_A + WK * NB;
+    static_assert(F % W == 0, "fragments must split evenly over the waves");
+    static_assert(DA * NB >= 5 && DA <= DG_A, "the interleave needs five MFMAs per step");
+    constexpr int FW = F / W;
+    constexpr int STEP_V4 = F * FRAG_V4;
+    constexpr int NM = DA * NB;  // MFMAs per step
+    __shared__ v4i ring[STAGES * STEP_V4];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int per = gridDim.x / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (t >= ntiles) return;
+    const int tk = t % ntk, tr = t / ntk;
+    const int wr = w / WK, wk = w % WK;
+    const v4i *src[FW];
+    long long step[FW];
+#pragma unroll
+    for (int u = 0; u < FW; ++u) {
+        const int f = w + u * W;
+        if (f < WR * DG_A) {
+            src[u] = A + (size_t)(tr * WR + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4 + lane;
+            step[u] = DG_A * FRAG_V4;
+        } else {
+            const int fb = f - WR * DG_A;
+            src[u] = B + (size_t)(tk * WK + fb / NB) * b_stride + (fb % NB) * FRAG_V4 + lane;
+            step[u] = NB * FRAG_V4;
+        }
+    }
+    auto issue = [&](int ks) {
+        v4i *slot = ring + (ks % STAGES) * STEP_V4;
+#pragma unroll
+        for (int u = 0; u < FW; ++u) {
+            const int f = w + u * W;
+            __builtin_amdgcn_global_load_lds((glb_void_t *)(src[u] + (size_t)ks * step[u]),
+                                             (lds_void_t *)(slot + f * FRAG_V4), 16, 0, 0);
+        }
+    };
+    const int a_off = wr * DG_A * FRAG_V4 + lane;
+    const int b_off = (WR * DG_A + wk * NB) * FRAG_V4 + lane;
+    v16i acc[DA][NB];
+#pragma unroll
+    for (int d = 0; d < DA; ++d)
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+#pragma unroll
+    for (int p = 0; p < STAGES; ++p)
+        if (p < ksteps) issue(p);
+    if (ksteps >= STAGES)
+        wait_vmcnt<(STAGES - 1) * FW>();
+    else
+        wait_vmcnt<0>();
+    asm volatile("s_barrier" ::: "memory");
+    v4i fa0[DA], fb0[NB], fa1[DA], fb1[NB];
+#pragma unroll
+    for (int d = 0; d < DA; ++d) fa0[d] = ring[a_off + d * FRAG_V4];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) fb0[e] = ring[b_off + e * FRAG_V4];
+    // One step.  In the last step the reads of "step ksteps" fetch a slot no
+    // one loads any more: in bounds, and never used.  Every wave runs the same
+    // number of barriers.
+    auto body = [&](auto main, int ks, v4i (&fa)[DA], v4i (&fb)[NB], v4i (&na)[DA], v4i (&nb)[NB]) {
+        constexpr bool M = decltype(main)::value;
+        const v4i *nx = ring + ((ks + 1) % STAGES) * STEP_V4;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            acc[i / NB][i % NB] =
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i / NB], fb[i % NB], acc[i / NB][i % NB], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (i == 1) {
+                if constexpr (M)
+                    wait_vmcnt<(STAGES - 2) * FW>();
+                else
+                    wait_vmcnt<0>();
+                asm volatile("s_barrier" ::: "memory");
+            } else if (i == 2) {
+                if constexpr (M)
+                    issue(ks + STAGES);
+                else if (ks + STAGES < ksteps)
+                    issue(ks + STAGES);
+            } else if (i == 3) {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) nb[e] = nx[b_off + e * FRAG_V4];
+            } else if (i >= 4) {
+                // row fragment i - 4; the last MFMA also takes any left over
+#pragma unroll
+                for (int d = 0; d < DA; ++d)
+                    if (d == i - 4 || (i == NM - 1 && d > i - 4)) na[d] = nx[a_off + d * FRAG_V4];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    using Main = std::integral_constant<bool, true>;
+    using Tail = std::integral_constant<bool, false>;
+    int ks = 0;
+    // main: steps ks and ks + 1 both still load step + STAGES
+    for (; ks + 1 + STAGES < ksteps; ks += 2) {
+        body(Main{}, ks, fa0, fb0, fa1, fb1);
+        body(Main{}, ks + 1, fa1, fb1, fa0, fb0);
+    }
+    for (; ks < ksteps; ks += 2) {
+        body(Tail{}, ks, fa0, fb0, fa1, fb1);
+        if (ks + 1 < ksteps) body(Tail{}, ks + 1, fa1, fb1, fa0, fb0);
+    }
+    const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
+    if (key >= S) return;
+    const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
+    const double qinv = 1.0 / (double)q;
+    const long long qq = q;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = row0 + (i >> 2) * 8 + (i & 3);
+        if (row >= R) continue;
+        long long tot = 0;
+#pragma unroll
+        for (int e = NB - 1; e >= 0; --e) {
+            long long T = 0;
+#pragma unroll
+            for (int d = 0; d < DA; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
+            long long m = T - (long long)floor((double)T * qinv) * qq;
+            m += (m < 0) ? qq : 0;
+            m -= (m >= qq) ? qq : 0;
+            tot = tot * 256 + m;
+            tot -= (long long)floor((double)tot * qinv) * qq;
+            tot += (tot < 0) ? qq : 0;
+            tot -= (tot >= qq) ? qq : 0;
+        }
+        D[(size_t)row * S + key] = (uint32_t)tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Hybrid variant: one row block shared by WK waves, one key block per wave.
 // The row digits (4 fragments per k step, used by every wave) go through the
 // LDS ring by global_load_lds; each wave's key digits (used by that wave only)
@@ -381,6 +541,8 @@ __global__ __launch_bounds__(64 * WK, 2) void k_contract_i8_hyb(const v4i *__res
     }
 }
 
+constexpr int kThreeDigitMax = 127 * (1 + 256 + 65536);  // 8355711
+
 __device__ __forceinline__ int digit_split(int &x)
 {
     const int d = ((x + 128) & 255) - 128;
@@ -402,6 +564,13 @@ __global__ void k_rows_digits(const int32_t *__restrict__ a32, int cap, int ldr,
     int v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) v[u] = (k + u < ldr) ? a32[((size_t)s * cap + row) * ldr + k + u] : 0;
+    // limb 0 (q < 2^24): the representative in [-8421504, 8355711], the range
+    // of three balanced base-256 digits, so digit 3 is zero
+    if (s == 0) {
+        const int q0 = (int)qmod(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] -= (v[u] > kThreeDigitMax) ? q0 : 0;
+    }
     const int ks = k >> 5, kk = k & 31;
     const int ln = (kk >> 4) * 32 + (row & 31), j = kk & 15;
     int8_t *base = dig + s * limb_bytes + ((size_t)(row >> 5) * ksteps + ks) * DG_A * 1024 + ln * 16 + j;
@@ -475,18 +644,21 @@ int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int
 }
 
 // Kernel variants the launcher can run: row blocks x key blocks per block
-// tile, and the LDS ring depth of the pipelined kernel (0: the register-staged
-// double-buffered kernel).  A pipelined variant needs its fragments to split
-// evenly over its waves for the given key digits.
-// kpb: k steps per barrier of the pipelined kernel; -1 marks the hybrid
-// kernel (row digits through LDS, key digits straight to registers).
+// tile, the LDS ring depth (0: the register-staged double-buffered kernel),
+// the kernel, and the row digits the MFMAs use (3: limb 0 only).
+enum I8Kind { I8_REG, I8_RING, I8_RING_K2, I8_HYBRID, I8_INTERLEAVED };
 struct I8Variant {
-    int wr, wk, stages, kpb;
+    int wr, wk, stages, kind, da;
 };
-static const I8Variant kVariants[] = {{2, 4, 0, 1}, {2, 2, 0, 1}, {4, 2, 0, 1}, {1, 4, 0, 1}, {4, 1, 0, 1},
-                                      {2, 2, 4, 1}, {1, 4, 4, 1}, {2, 4, 4, 1}, {2, 2, 3, 1}, {1, 4, 3, 1},
-                                      {2, 2, 2, 2}, {2, 2, 3, 2}, {1, 4, 2, 2}, {2, 4, 2, 2},
-                                      {1, 4, 3, -1}, {1, 4, 4, -1}, {1, 2, 3, -1}, {1, 4, 2, -1}};
+static const I8Variant kVariants[] = {
+    {2, 2, 0, I8_REG, 4},         {1, 4, 0, I8_REG, 4},         {2, 4, 0, I8_REG, 4},
+    {2, 2, 4, I8_RING, 4},        {1, 4, 4, I8_RING, 4},        {2, 2, 3, I8_RING, 4},
+    {1, 4, 3, I8_RING, 4},        {2, 4, 2, I8_RING_K2, 4},     {1, 4, 2, I8_RING_K2, 4},
+    {1, 4, 3, I8_HYBRID, 4},      {1, 4, 2, I8_HYBRID, 4},      {2, 2, 3, I8_INTERLEAVED, 4},
+    {2, 2, 4, I8_INTERLEAVED, 4}, {1, 4, 4, I8_INTERLEAVED, 4}, {2, 4, 4, I8_INTERLEAVED, 4},
+    {2, 2, 3, I8_INTERLEAVED, 3}, {2, 2, 4, I8_INTERLEAVED, 3}, {1, 4, 4, I8_INTERLEAVED, 3},
+    {2, 4, 4, I8_INTERLEAVED, 3}, {1, 4, 3, I8_INTERLEAVED, 4}, {1, 4, 3, I8_INTERLEAVED, 3},
+};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 int i8_shape_count() { return kNumVariants; }
@@ -496,16 +668,43 @@ void i8_shape(int v, int *wr, int *wk)
     *wk = kVariants[v].wk;
 }
 int i8_variant_stages(int v) { return kVariants[v].stages; }
-bool i8_variant_ok(int v, int nb)
+// "WRxWK" for the register-staged kernel; "WRxWK/STAGES" plus "k2" (two k
+// steps per barrier), "h" (hybrid) or "i" (interleaved; "i3": three row
+// digits) for the others
+const char *i8_variant_label(int v)
+{
+    struct Labels {
+        char s[kNumVariants][24];
+        Labels()
+        {
+            for (int i = 0; i < kNumVariants; ++i) {
+                const I8Variant &x = kVariants[i];
+                const char *suffix = x.kind == I8_RING_K2       ? "k2"
+                                     : x.kind == I8_HYBRID      ? "h"
+                                     : x.kind == I8_INTERLEAVED ? (x.da == 3 ? "i3" : "i")
+                                                                : "";
+                if (x.kind == I8_REG)
+                    snprintf(s[i], sizeof s[i], "%dx%d", x.wr, x.wk);
+                else
+                    snprintf(s[i], sizeof s[i], "%dx%d/%d%s", x.wr, x.wk, x.stages, suffix);
+            }
+        }
+    };
+    static const Labels labels;  // initialised once, thread-safe
+    return v >= 0 && v < kNumVariants ? labels.s[v] : "?";
+}
+// Variants instantiated for nb key digits on limb s (three row digits only on
+// limb 0, whose row digit store holds a zero digit 3).  With three key digits:
+// the register-staged kernels and the 1x4 tiles of the others (the 1x4/3
+// hybrid only).
+bool i8_variant_ok(int v, int nb, int s)
 {
     if (v < 0 || v >= kNumVariants || nb < 2 || nb > 3) return false;
     const I8Variant &x = kVariants[v];
-    if (!x.stages) return true;
-    if (x.kpb < 0) return nb == 2 || v == 14;  // hybrid: all with two key digits, one with three
-    const int W = x.wr * x.wk, F = x.wr * DG_A + x.wk * nb;
-    if (F % W) return false;
-    if (nb == 3) return v == 6 || v == 9 || v == 12;  // the instantiated three-digit kernels
-    return true;
+    if (x.da == 3 && s != 0) return false;
+    if (nb == 2 || x.kind == I8_REG) return true;
+    if (x.kind == I8_HYBRID) return x.stages == 3;
+    return x.wr == 1 && x.kind != I8_INTERLEAVED;
 }
 
 // A: row digit store of one limb at row r0 (r0 a multiple of 32; the store
@@ -515,8 +714,9 @@ bool i8_variant_ok(int v, int nb)
 int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
                        uint32_t *D, hipStream_t st, int variant)
 {
+
     if (R <= 0 || S <= 0) return 0;
-    if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 17) || !i8_variant_ok(variant, nb)) {
+    if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 17) || !i8_variant_ok(variant, nb, s)) {
         ipfe_set_error("int8 contraction: %d k steps (row store %d), variant %d, %d key digits", ksteps, ksteps_a,
                        variant, nb);
         return -1;
@@ -536,46 +736,51 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 #define CI8D(wr, wk, n, stg, kpb)                                                                                  \
     hipLaunchKernelGGL((k_contract_i8_dma<wr, wk, n, stg, kpb>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, \
                        ntk, ntiles)
+#define CI8I(wr, wk, n, stg, da)                                                                                     \
+    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D,  \
+                       ntk, ntiles)
 #define CI8H(wk, n, stg)                                                                                             \
     hipLaunchKernelGGL((k_contract_i8_hyb<wk, n, stg>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk,    \
                        ntiles)
     if (nb == 2) {
         switch (variant) {
-        case 0: CI8(2, 4, 2); break;
-        case 1: CI8(2, 2, 2); break;
-        case 2: CI8(4, 2, 2); break;
-        case 3: CI8(1, 4, 2); break;
-        case 4: CI8(4, 1, 2); break;
-        case 5: CI8D(2, 2, 2, 4, 1); break;
-        case 6: CI8D(1, 4, 2, 4, 1); break;
-        case 7: CI8D(2, 4, 2, 4, 1); break;
-        case 8: CI8D(2, 2, 2, 3, 1); break;
-        case 9: CI8D(1, 4, 2, 3, 1); break;
-        case 10: CI8D(2, 2, 2, 2, 2); break;
-        case 11: CI8D(2, 2, 2, 3, 2); break;
-        case 12: CI8D(1, 4, 2, 2, 2); break;
-        case 13: CI8D(2, 4, 2, 2, 2); break;
-        case 14: CI8H(4, 2, 3); break;
-        case 15: CI8H(4, 2, 4); break;
-        case 16: CI8H(2, 2, 3); break;
-        default: CI8H(4, 2, 2); break;  // variant 17
+        case 0: CI8(2, 2, 2); break;
+        case 1: CI8(1, 4, 2); break;
+        case 2: CI8(2, 4, 2); break;
+        case 3: CI8D(2, 2, 2, 4, 1); break;
+        case 4: CI8D(1, 4, 2, 4, 1); break;
+        case 5: CI8D(2, 2, 2, 3, 1); break;
+        case 6: CI8D(1, 4, 2, 3, 1); break;
+        case 7: CI8D(2, 4, 2, 2, 2); break;
+        case 8: CI8D(1, 4, 2, 2, 2); break;
+        case 9: CI8H(4, 2, 3); break;
+        case 10: CI8H(4, 2, 2); break;
+        case 11: CI8I(2, 2, 2, 3, 4); break;
+        case 12: CI8I(2, 2, 2, 4, 4); break;
+        case 13: CI8I(1, 4, 2, 4, 4); break;
+        case 14: CI8I(2, 4, 2, 4, 4); break;
+        case 15: CI8I(2, 2, 2, 3, 3); break;
+        case 16: CI8I(2, 2, 2, 4, 3); break;
+        case 17: CI8I(1, 4, 2, 4, 3); break;
+        case 18: CI8I(2, 4, 2, 4, 3); break;
+        case 19: CI8I(1, 4, 2, 3, 4); break;
+        default: CI8I(1, 4, 2, 3, 3); break;  // variant 20
         }
     } else {
         switch (variant) {
-        case 0: CI8(2, 4, 3); break;
-        case 1: CI8(2, 2, 3); break;
-        case 2: CI8(4, 2, 3); break;
-        case 3: CI8(1, 4, 3); break;
-        case 4: CI8(4, 1, 3); break;
-        case 6: CI8D(1, 4, 3, 4, 1); break;
-        case 9: CI8D(1, 4, 3, 3, 1); break;
-        case 12: CI8D(1, 4, 3, 2, 2); break;
-        default: CI8H(4, 3, 3); break;  // variant 14
+        case 0: CI8(2, 2, 3); break;
+        case 1: CI8(1, 4, 3); break;
+        case 2: CI8(2, 4, 3); break;
+        case 4: CI8D(1, 4, 3, 4, 1); break;
+        case 6: CI8D(1, 4, 3, 3, 1); break;
+        case 8: CI8D(1, 4, 3, 2, 2); break;
+        default: CI8H(4, 3, 3); break;  // variant 9
         }
     }
 #undef CI8
 #undef CI8D
 #undef CI8H
+#undef CI8I
     (void)ST;
     HIP_OK(hipGetLastError());
     return 0;

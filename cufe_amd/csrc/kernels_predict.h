@@ -41,7 +41,8 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 int i8_shape_count();
 void i8_shape(int variant, int *wr, int *wk);
 int i8_variant_stages(int variant);
-bool i8_variant_ok(int variant, int nb);
+const char *i8_variant_label(int variant);
+bool i8_variant_ok(int variant, int nb, int s);
 // largest key coefficients the two- and three-digit key splits hold: the top
 // digit (|b| + 128 (1 + 256 + ...)) / 256^(nb-1) must stay within [-128, 127]
 constexpr unsigned int I8_KEY_MAX2 = 32512;

@@ -54,9 +54,13 @@ int ilog2(int x)
 int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
                       uint32_t *D, hipStream_t st)
 {
+    // a forced variant runs on the limbs it is valid for (three-row-digit
+    // variants: limb 0 only); the others are tuned
     const char *env = getenv("CUFE_I8_VARIANT");
-    if (env && *env) return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, atoi(env));
-    const auto key = std::make_tuple(S, ilog2(R), ksteps * 4 + nb);
+    if (env && *env && i8_variant_ok(atoi(env), nb, s))
+        return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, atoi(env));
+    // limb 0 has its own variant set (three row digits)
+    const auto key = std::make_tuple(S, ilog2(R), (ksteps * 4 + nb) * 2 + (s == 0));
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
@@ -71,22 +75,25 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
     int best = 0;
     float best_ms = 1e30f;
     char line[512];
-    int off = snprintf(line, sizeof line, "S=%d R=%d ksteps=%d key digits=%d:", S, R, ksteps, nb);
+    int off = snprintf(line, sizeof line, "S=%d R=%d ksteps=%d key digits=%d limb%s:", S, R, ksteps, nb,
+                       s == 0 ? " 0" : "s 1-2");
     for (int v = 0; v < i8_shape_count(); ++v) {
-        if (!i8_variant_ok(v, nb)) continue;
-        HIP_OK(hipEventRecord(e0, st));
-        if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v)) return -1;
-        HIP_OK(hipEventRecord(e1, st));
-        HIP_OK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        if (!i8_variant_ok(v, nb, s)) continue;
+        // the faster of two launches: one slow launch (a clock change, another
+        // stream's work) does not decide the choice
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_OK(hipEventRecord(e0, st));
+            if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v)) return -1;
+            HIP_OK(hipEventRecord(e1, st));
+            HIP_OK(hipEventSynchronize(e1));
+            float t = 0;
+            HIP_OK(hipEventElapsedTime(&t, e0, e1));
+            ms = t < ms ? t : ms;
+        }
         int wr, wk;
         i8_shape(v, &wr, &wk);
-        const int stg = i8_variant_stages(v);
-        if (stg)
-            off += snprintf(line + off, sizeof line - off, " %dx%d/%d %.3f", wr, wk, stg, ms);
-        else
-            off += snprintf(line + off, sizeof line - off, " %dx%d %.3f", wr, wk, ms);
+        off += snprintf(line + off, sizeof line - off, " %s %.3f", i8_variant_label(v), ms);
         times[v] = ms;
         if (ms < best_ms) {
             best_ms = ms;
@@ -106,8 +113,7 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
     (void)hipEventDestroy(e1);
     int wr, wk;
     i8_shape(best, &wr, &wk);
-    snprintf(line + off, sizeof line - off, " ms -> variant %d (%dx%d%s)\n", best, wr, wk,
-             i8_variant_stages(best) ? ", LDS ring" : "");
+    snprintf(line + off, sizeof line - off, " ms -> variant %d (%s)\n", best, i8_variant_label(best));
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = best;
     g_tune_log += line;
@@ -352,7 +358,7 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
     }
     const int chunk = std::max(64, scratch_rows / 64 * 64);
     if (sc.reserve(std::min(chunk, ((nrows + 63) / 64) * 64), S)) return -1;
-    if (tm) tm->ms[0] = tm->ms[1] = tm->ms[2] = 0;
+    if (tm) tm->ms[0] = tm->ms[1] = tm->ms[2] = tm->limb_ms[0] = tm->limb_ms[1] = tm->limb_ms[2] = 0;
     if (launch_fill_i32(sc.flag.get(), 0, 1, st)) return -1;
     for (int r0 = row0; r0 < row0 + nrows; r0 += chunk) {
         const int R = std::min(chunk, row0 + nrows - r0);
@@ -377,6 +383,7 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
             else
                 rc = launch_contract(As, Bs, R, S, m.kkey, rs.ldr, m.kkey, s, m.kb[s], Ds, st);
             if (rc) return -1;
+            if (tm && s < 2) HIP_OK(hipEventRecord(tm->ev[4 + s], st));
         }
         if (tm) HIP_OK(hipEventRecord(tm->ev[1], st));
         if (m.fused) {
@@ -405,6 +412,13 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
                 tm->ms[0] += a;
                 tm->ms[1] += b;
                 tm->ms[2] += d;
+                float l0 = 0, l1 = 0, l2 = 0;
+                HIP_OK(hipEventElapsedTime(&l0, tm->ev[0], tm->ev[4]));
+                HIP_OK(hipEventElapsedTime(&l1, tm->ev[4], tm->ev[5]));
+                HIP_OK(hipEventElapsedTime(&l2, tm->ev[5], tm->ev[1]));
+                tm->limb_ms[0] += l0;
+                tm->limb_ms[1] += l1;
+                tm->limb_ms[2] += l2;
             }
         }
     }
@@ -491,7 +505,8 @@ struct cufe_ctx {
     RowStore rows;
     int nrows = 0;
     Scratch scratch;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
+    float limb_ms[3] = {};  // contraction per limb in the last predict call
 };
 
 namespace ipfe {
@@ -656,6 +671,8 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
     const int chunk_rows = (int)std::max<size_t>(64, ((size_t)1 << 30) / (size_t)c->model.S);
     PredictTimers tm;
     for (int i = 0; i < 4; ++i) tm.ev[i] = c->ev[i];
+    tm.ev[4] = c->ev[8];
+    tm.ev[5] = c->ev[9];
     HIP_OK(hipEventRecord(c->ev[4], st));
     if (predict_rows(c->model, c->rows, row0, nrows, c->scratch, chunk_rows, contract_mode(), labels, kvalues, &tm, st))
         return -1;
@@ -665,6 +682,7 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
     HIP_OK(hipEventSynchronize(c->ev[5]));
     float tot = 0;
     HIP_OK(hipEventElapsedTime(&tot, c->ev[4], c->ev[5]));
+    for (int s = 0; s < 3; ++s) c->limb_ms[s] = tm.limb_ms[s];
     if (ms) {
         ms[0] = tm.ms[0];
         ms[1] = tm.ms[1];
@@ -676,6 +694,18 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
                        "this needs the full-polynomial path (keys must be shorter than the slot count)");
         return -1;
     }
+    return 0;
+}
+
+// Contraction time per limb (ms, summed over chunks) of the last
+// cufe_ctx_predict call.
+extern "C" int cufe_ctx_contract_limb_ms(const cufe_ctx *c, float *ms)
+{
+    if (!c || !ms) {
+        ipfe_set_error("cufe_ctx_contract_limb_ms: null argument");
+        return -1;
+    }
+    for (int s = 0; s < 3; ++s) ms[s] = c->limb_ms[s];
     return 0;
 }
 
@@ -695,12 +725,14 @@ extern "C" int cufe_ctx_model_info(const cufe_ctx *c, int *kb, unsigned int *max
 }
 
 // Number of int8 contraction variants, and whether variant v runs with nb key
-// digits (CUFE_I8_VARIANT=v forces one).
+// digits on limb 0 (every variant valid on limbs 1-2 also runs on limb 0;
+// CUFE_I8_VARIANT=v forces one on the limbs it runs on).
 extern "C" int cufe_contract_variants(int nb, unsigned int *valid_mask)
 {
+    static_assert(sizeof(unsigned int) * 8 >= 32, "mask of 32 variants");
     unsigned int m = 0;
     for (int v = 0; v < i8_shape_count() && v < 32; ++v)
-        if (i8_variant_ok(v, nb)) m |= 1u << v;
+        if (i8_variant_ok(v, nb, 0)) m |= 1u << v;
     if (valid_mask) *valid_mask = m;
     return i8_shape_count();
 }

@@ -75,9 +75,13 @@ struct Scratch {
 // (optional) kvalues copied to host memory asynchronously on st.  Chunked so
 // that the scratch stays within `scratch_rows` rows.  ms (optional, 3 floats,
 // only when timing events are passed): contraction / key-value / vote time.
+// ev[0..3]: before the contraction, after it, after the key values, after
+// the vote; ev[4], ev[5]: after the contraction of limb 0 and of limb 1.
+// ms: contraction, key values, vote; limb_ms: the contraction per limb.
 struct PredictTimers {
-    hipEvent_t ev[4];
+    hipEvent_t ev[6];
     float ms[3];
+    float limb_ms[3];
 };
 int predict_rows(const Model &m, const RowStore &rs, int r0, int R, Scratch &sc, int scratch_rows, int mode,
                  double *labels_host, uint32_t *kval_host, PredictTimers *tm, hipStream_t st);

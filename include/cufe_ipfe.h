@@ -178,6 +178,10 @@ int cufe_ctx_rows_count(const cufe_ctx *ctx);
  * [nrows][model l] = the decrypted inner products; ms (optional, 4 floats) =
  * device milliseconds of contraction, key-value epilogue, vote, total. */
 int cufe_ctx_predict(cufe_ctx *ctx, int row0, int nrows, double *labels, uint32_t *kvalues, float *ms);
+/* Contraction time per limb (ms[3], summed over row chunks) of the last
+ * cufe_ctx_predict call (limb 0 may run a cheaper kernel: its residues fit
+ * three int8 digits). */
+int cufe_ctx_contract_limb_ms(const cufe_ctx *ctx, float *ms);
 /* Reduction interval of the fp64 contraction and the largest balanced
  * functional-key coefficient, per limb. */
 int cufe_ctx_model_info(const cufe_ctx *ctx, int *kb, unsigned int *maxabs);

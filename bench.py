@@ -473,7 +473,9 @@ def main():
         "roofline": roof,
         "autotune": cf.autotune_report().strip().split("\n"),
         "phases": {"setup_s": t_setup, "encrypt_s": t_enc, "encrypt_rows_per_s": R / t_enc,
-                   "keygen_ms": t_keygen * 1e3, "last_step_device_ms": {"contract": ms[0], "kvalue": ms[1],
+                   "keygen_ms": t_keygen * 1e3,
+                   "contract_ms_per_limb": [round(t / args.steps, 4) for t in limb_ms],
+                   "last_step_device_ms": {"contract": ms[0], "kvalue": ms[1],
                                                                          "vote": ms[2], "total": ms[3]},
                    "predictions_per_s_with_encryption": R / (t_enc + ms_per_step / 1e3)},
         "decryption_check": {"pairs": chk * S, "fraction_equal_mod_p": dec_ok, "zero_decrypted_as_p": dec_p},

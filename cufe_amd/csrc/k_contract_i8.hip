@@ -24,6 +24,7 @@
 
 #include <type_traits>
 
+#include "crt_dev.h"
 #include "kernels_predict.h"
 
 namespace ipfe {
@@ -33,6 +34,52 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int DG_A = 4;  // digits per row value (key values: NB = 2 or 3)
 constexpr int FRAG_V4 = 64;        // one fragment: 64 lanes x 16 bytes
+
+// Where a contraction writes: the limb's residues D[R][S], or -- for the last
+// limb, with kval set -- the rounded decryption of every pair, combined by CRT
+// with the residues d0, d1 of limbs 0 and 1 (what k_crt_kvalue computes from
+// three D planes, without writing or re-reading this limb's plane).
+struct I8Out {
+    uint32_t *D;
+    const uint32_t *d0, *d1;
+    uint32_t *kval;
+};
+
+// Epilogue shared by the kernels: lane (row0, key) holds acc[d][e][i] =
+// P_de for row row0 + 8 (i / 4) + i mod 4.  T_e = sum_d 256^d P_de (|T_e| <
+// 2^56) is reduced mod q, then the residues are combined with the weights
+// 256^e; the double quotients are off by at most one and corrected.
+template <int DA, int NB>
+__device__ __forceinline__ void i8_store(const v16i (&acc)[DA][NB], int row0, int key, int R, int S, uint32_t q,
+                                         const I8Out &out)
+{
+    const double qinv = 1.0 / (double)q;
+    const long long qq = q;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = row0 + (i >> 2) * 8 + (i & 3);
+        if (row >= R) continue;
+        long long tot = 0;
+#pragma unroll
+        for (int e = NB - 1; e >= 0; --e) {
+            long long T = 0;
+#pragma unroll
+            for (int d = 0; d < DA; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
+            long long m = T - (long long)floor((double)T * qinv) * qq;
+            m += (m < 0) ? qq : 0;
+            m -= (m >= qq) ? qq : 0;
+            tot = tot * 256 + m;
+            tot -= (long long)floor((double)tot * qinv) * qq;
+            tot += (tot < 0) ? qq : 0;
+            tot -= (tot >= qq) ? qq : 0;
+        }
+        const size_t idx = (size_t)row * S + key;
+        if (out.kval)
+            out.kval[idx] = crt_round(crt_garner(out.d0[idx], out.d1[idx], (uint32_t)tot));
+        else
+            out.D[idx] = (uint32_t)tot;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Block tile: WR row blocks x WK key blocks of 32, one wave per 32 x 32 output
@@ -54,7 +101,7 @@ template <int WR, int WK, int NB>
 __global__ __launch_bounds__(64 * WR * WK, (i8_min_waves<WR, WK, NB>())) void k_contract_i8(const v4i *__restrict__ A, const v4i *__restrict__ B,
                                                                int R, int S, int ksteps, long long a_stride,
                                                                long long b_stride, uint32_t q,
-                                                               uint32_t *__restrict__ D, int ntk, int ntiles)
+                                                               I8Out out, int ntk, int ntiles)
 {
     constexpr int NT = 64 * WR * WK;
     constexpr int SA = WR * DG_A * FRAG_V4, SB = WK * NB * FRAG_V4;  // v4i per k step
@@ -122,30 +169,7 @@ __global__ __launch_bounds__(64 * WR * WK, (i8_min_waves<WR, WK, NB>())) void k_
     const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
     if (key >= S) return;
     const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
-    const double qinv = 1.0 / (double)q;
-    const long long qq = q;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = row0 + (i >> 2) * 8 + (i & 3);
-        if (row >= R) continue;
-        long long tot = 0;
-#pragma unroll
-        for (int e = NB - 1; e >= 0; --e) {
-            long long T = 0;
-#pragma unroll
-            for (int d = 0; d < DG_A; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
-            // |T| < 2^56 and then |tot| < 2^40: the double quotient is off by
-            // at most one
-            long long m = T - (long long)floor((double)T * qinv) * qq;
-            m += (m < 0) ? qq : 0;
-            m -= (m >= qq) ? qq : 0;
-            tot = tot * 256 + m;
-            tot -= (long long)floor((double)tot * qinv) * qq;
-            tot += (tot < 0) ? qq : 0;
-            tot -= (tot >= qq) ? qq : 0;
-        }
-        D[(size_t)row * S + key] = (uint32_t)tot;
-    }
+    i8_store<DG_A, NB>(acc, row0, key, R, S, q, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -173,7 +197,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
                                                                      const v4i *__restrict__ B, int R, int S,
                                                                      int ksteps, long long a_stride,
                                                                      long long b_stride, uint32_t q,
-                                                                     uint32_t *__restrict__ D, int ntk, int ntiles)
+                                                                     I8Out out, int ntk, int ntiles)
 {
     constexpr int W = WR * WK;
     constexpr int F = WR * DG_A + WK * NB;  // fragments per k step
@@ -260,28 +284,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
     const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
     if (key >= S) return;
     const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
-    const double qinv = 1.0 / (double)q;
-    const long long qq = q;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = row0 + (i >> 2) * 8 + (i & 3);
-        if (row >= R) continue;
-        long long tot = 0;
-#pragma unroll
-        for (int e = NB - 1; e >= 0; --e) {
-            long long T = 0;
-#pragma unroll
-            for (int d = 0; d < DG_A; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
-            long long m = T - (long long)floor((double)T * qinv) * qq;
-            m += (m < 0) ? qq : 0;
-            m -= (m >= qq) ? qq : 0;
-            tot = tot * 256 + m;
-            tot -= (long long)floor((double)tot * qinv) * qq;
-            tot += (tot < 0) ? qq : 0;
-            tot -= (tot >= qq) ? qq : 0;
-        }
-        D[(size_t)row * S + key] = (uint32_t)tot;
-    }
+    i8_store<DG_A, NB>(acc, row0, key, R, S, q, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
                                                                     const v4i *__restrict__ B, int R, int S,
                                                                     int ksteps, long long a_stride,
                                                                     long long b_stride, uint32_t q,
-                                                                    uint32_t *__restrict__ D, int ntk, int ntiles)
+                                                                    I8Out out, int ntk, int ntiles)
 {
     constexpr int W = WR * WK;
     constexpr int F = WR * DG_A + WK * NB;
@@ -416,28 +419,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
     const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
     if (key >= S) return;
     const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
-    const double qinv = 1.0 / (double)q;
-    const long long qq = q;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = row0 + (i >> 2) * 8 + (i & 3);
-        if (row >= R) continue;
-        long long tot = 0;
-#pragma unroll
-        for (int e = NB - 1; e >= 0; --e) {
-            long long T = 0;
-#pragma unroll
-            for (int d = 0; d < DA; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
-            long long m = T - (long long)floor((double)T * qinv) * qq;
-            m += (m < 0) ? qq : 0;
-            m -= (m >= qq) ? qq : 0;
-            tot = tot * 256 + m;
-            tot -= (long long)floor((double)tot * qinv) * qq;
-            tot += (tot < 0) ? qq : 0;
-            tot -= (tot >= qq) ? qq : 0;
-        }
-        D[(size_t)row * S + key] = (uint32_t)tot;
-    }
+    i8_store<DA, NB>(acc, row0, key, R, S, q, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -453,7 +435,7 @@ template <int WK, int NB, int STAGES>
 __global__ __launch_bounds__(64 * WK, 2) void k_contract_i8_hyb(const v4i *__restrict__ A,
                                                                const v4i *__restrict__ B, int R, int S,
                                                                int ksteps, long long a_stride, long long b_stride,
-                                                               uint32_t q, uint32_t *__restrict__ D, int ntk,
+                                                               uint32_t q, I8Out out, int ntk,
                                                                int ntiles)
 {
     static_assert(DG_A % WK == 0, "row fragments must split evenly over the waves");
@@ -517,28 +499,7 @@ __global__ __launch_bounds__(64 * WK, 2) void k_contract_i8_hyb(const v4i *__res
     const int key = tk * (32 * WK) + w * 32 + (lane & 31);
     if (key >= S) return;
     const int row0 = tr * 32 + (lane >> 5) * 4;
-    const double qinv = 1.0 / (double)q;
-    const long long qq = q;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = row0 + (i >> 2) * 8 + (i & 3);
-        if (row >= R) continue;
-        long long tot = 0;
-#pragma unroll
-        for (int e = NB - 1; e >= 0; --e) {
-            long long T = 0;
-#pragma unroll
-            for (int d = 0; d < DG_A; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
-            long long m = T - (long long)floor((double)T * qinv) * qq;
-            m += (m < 0) ? qq : 0;
-            m -= (m >= qq) ? qq : 0;
-            tot = tot * 256 + m;
-            tot -= (long long)floor((double)tot * qinv) * qq;
-            tot += (tot < 0) ? qq : 0;
-            tot -= (tot >= qq) ? qq : 0;
-        }
-        D[(size_t)row * S + key] = (uint32_t)tot;
-    }
+    i8_store<DG_A, NB>(acc, row0, key, R, S, q, out);
 }
 
 constexpr int kThreeDigitMax = 127 * (1 + 256 + 65536);  // 8355711
@@ -712,7 +673,8 @@ bool i8_variant_ok(int v, int nb, int s)
 // of one limb with nb digits, ksteps k steps (<= ksteps_a), keys padded to a
 // multiple of 128.  variant: index into kVariants.
 int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
-                       uint32_t *D, hipStream_t st, int variant)
+                       uint32_t *D_out, hipStream_t st, int variant, const uint32_t *crt_d0, const uint32_t *crt_d1,
+                       uint32_t *kval)
 {
 
     if (R <= 0 || S <= 0) return 0;
@@ -724,13 +686,23 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
     const int WR = kVariants[variant].wr, WK = kVariants[variant].wk, ST = kVariants[variant].stages;
     // row tiles never reach past the 128 padding rows; key tiles stay within
     // the 128-key padding
-    const int ntk = (S + 32 * WK - 1) / (32 * WK), ntr = (R + 32 * WR - 1) / (32 * WR);
+    const int key_tile = 32 * WK, row_tile = 32 * WR;
+    if (key_tile > 128 || row_tile > 128) {
+        ipfe_set_error("int8 contraction: variant %d tiles past the 128-row / 128-key padding", variant);
+        return -1;
+    }
+    const int ntk = (S + key_tile - 1) / key_tile, ntr = (R + row_tile - 1) / row_tile;
     const int ntiles = ntk * ntr;
     const dim3 grid(((ntiles + 7) / 8) * 8), block(64 * WR * WK);
     const v4i *a = reinterpret_cast<const v4i *>(A);
     const v4i *b = reinterpret_cast<const v4i *>(B);
     const long long as = (long long)ksteps_a * DG_A * FRAG_V4, bs = (long long)ksteps * nb * FRAG_V4;
     const uint32_t q = qmod(s);
+    if (kval && (s != 2 || !crt_d0 || !crt_d1)) {
+        ipfe_set_error("int8 contraction: the CRT epilogue runs on limb 2 with limbs 0 and 1 given");
+        return -1;
+    }
+    const I8Out D = {kval ? nullptr : D_out, crt_d0, crt_d1, kval};
 #define CI8(wr, wk, n) \
     hipLaunchKernelGGL((k_contract_i8<wr, wk, n>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
 #define CI8D(wr, wk, n, stg, kpb)                                                                                  \

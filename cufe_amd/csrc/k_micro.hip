@@ -3,6 +3,8 @@
 // HBM copy bandwidth and raw AES-256-CTR keystream rate on this GPU.
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../include/cufe_ipfe.h"
 #include "aes_dev.h"
 #include "ntt_core.h"
@@ -110,6 +112,27 @@ __global__ __launch_bounds__(256) void k_mb_mfma_i8(int *out, int iters)
     out[blockIdx.x * blockDim.x + threadIdx.x] = x;
 }
 
+// Four v_mfma_i32_32x32x32_i8 chains per wave on random operands, each MFMA
+// taking a different operand pair than the one before it: the int8 rate with
+// the data-dependent switching of real digits (constant operands draw less
+// power, and the clock stays higher).
+__global__ __launch_bounds__(256) void k_mb_mfma_i8_rand(const mb_v4i *__restrict__ src, int *out, int iters)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const mb_v4i a0 = src[4 * t], b0 = src[4 * t + 1], a1 = src[4 * t + 2], b1 = src[4 * t + 3];
+    mb_v16i c0 = {}, c1 = {}, c2 = {}, c3 = {};
+    for (int it = 0; it < iters; ++it) {
+        c0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c1, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c2, 0, 0, 0);
+        c3 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c3, 0, 0, 0);
+    }
+    int x = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x ^= c0[r] ^ c1[r] ^ c2[r] ^ c3[r];
+    out[t] = x;
+}
+
 // Four independent v_mfma_f64_16x16x4_f64 chains per wave.
 __global__ __launch_bounds__(256) void k_mb_mfma_f64(double *out, int iters)
 {
@@ -183,7 +206,34 @@ extern "C" int cufe_microbench(int kind, double *result)
     const int blocks = cus * 8;
     float ms = 0;
     double work = 0;
-    if (kind == 7) {
+    if (kind == 8) {
+        // random operands from a fixed xorshift stream; 8 launches of ~5 ms
+        // back to back so the clock settles to its sustained value
+        const size_t nthreads = (size_t)blocks * 256;
+        std::vector<uint32_t> h(nthreads * 16);
+        uint32_t x = 2463534242u;
+        for (auto &v : h) {
+            x ^= x << 13;
+            x ^= x >> 17;
+            x ^= x << 5;
+            v = x;
+        }
+        DevBuf<uint32_t> src, out;
+        if (src.alloc(h.size()) || out.alloc(nthreads)) return -1;
+        HIP_OK(hipMemcpy(src.get(), h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        const int iters = 4096, reps = 8;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_OK(hipEventRecord(a, st));
+            for (int r = 0; r < reps; ++r)
+                hipLaunchKernelGGL(k_mb_mfma_i8_rand, dim3(blocks), dim3(256), 0, st,
+                                   reinterpret_cast<const mb_v4i *>(src.get()), (int *)out.get(), iters);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipEventRecord(b, st));
+            HIP_OK(hipEventSynchronize(b));
+        }
+        HIP_OK(hipEventElapsedTime(&ms, a, b));
+        work = (double)blocks * 4 * iters * 4 * reps * (32.0 * 32 * 32 * 2);
+    } else if (kind == 7) {
         DevBuf<uint32_t> out;
         if (out.alloc((size_t)blocks * 256)) return -1;
         const int iters = 2048;
@@ -279,6 +329,6 @@ extern "C" int cufe_microbench(int kind, double *result)
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     const double s = ms * 1e-3;
-    *result = (kind == 0 || kind == 1 || kind == 5 || kind == 6) ? work / s / 1e12 : work / s / 1e9;
+    *result = (kind == 0 || kind == 1 || kind == 5 || kind == 6 || kind == 8) ? work / s / 1e12 : work / s / 1e9;
     return 0;
 }

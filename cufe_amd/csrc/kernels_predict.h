@@ -37,7 +37,8 @@ size_t keys_digit_limb_bytes(int spad, int ksteps, int nb);
 int launch_rows_digits(const int32_t *a32, int cap, int ldr, int r0, int R, int ksteps, int8_t *dig, hipStream_t st);
 int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int nb, int8_t *dig, hipStream_t st);
 int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
-                       uint32_t *D, hipStream_t st, int variant = 0);
+                       uint32_t *D, hipStream_t st, int variant = 0, const uint32_t *crt_d0 = nullptr,
+                       const uint32_t *crt_d1 = nullptr, uint32_t *kval = nullptr);
 int i8_shape_count();
 void i8_shape(int variant, int *wr, int *wk);
 int i8_variant_stages(int variant);

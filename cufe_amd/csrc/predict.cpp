@@ -51,26 +51,29 @@ int ilog2(int x)
 }
 }  // namespace
 
+// crt_d0, crt_d1, kval (limb 2 only): write the CRT-rounded key values
+// instead of this limb's residues.
 int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
-                      uint32_t *D, hipStream_t st)
+                      uint32_t *D, hipStream_t st, const uint32_t *crt_d0, const uint32_t *crt_d1, uint32_t *kval)
 {
     // a forced variant runs on the limbs it is valid for (three-row-digit
     // variants: limb 0 only); the others are tuned
     const char *env = getenv("CUFE_I8_VARIANT");
     if (env && *env && i8_variant_ok(atoi(env), nb, s))
-        return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, atoi(env));
+        return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, atoi(env), crt_d0, crt_d1, kval);
     // limb 0 has its own variant set (three row digits)
     const auto key = std::make_tuple(S, ilog2(R), (ksteps * 4 + nb) * 2 + (s == 0));
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);
-        if (it != g_tune.end()) return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, it->second);
+        if (it != g_tune.end())
+            return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, it->second, crt_d0, crt_d1, kval);
     }
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     // one untimed launch loads the code objects
-    if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, 0)) return -1;
+    if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, 0, crt_d0, crt_d1, kval)) return -1;
     std::vector<float> times(i8_shape_count(), -1.0f);
     int best = 0;
     float best_ms = 1e30f;
@@ -84,7 +87,7 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
         float ms = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
             HIP_OK(hipEventRecord(e0, st));
-            if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v)) return -1;
+            if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v, crt_d0, crt_d1, kval)) return -1;
             HIP_OK(hipEventRecord(e1, st));
             HIP_OK(hipEventSynchronize(e1));
             float t = 0;
@@ -363,16 +366,22 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
     for (int r0 = row0; r0 < row0 + nrows; r0 += chunk) {
         const int R = std::min(chunk, row0 + nrows - r0);
         if (tm) HIP_OK(hipEventRecord(tm->ev[0], st));
+        // every limb on the int8 path: the last one's epilogue also does the
+        // CRT and rounding of the fused keys' key values
+        const bool i8 = mode == 2 && m.i8 && r0 % 32 == 0;
+        const bool fuse_crt = i8 && m.fused && !getenv("CUFE_NO_FUSED_CRT");
         for (int s = 0; s < 3; ++s) {
             const int32_t *As = rs.A.get() + ((size_t)s * rs.cap + r0) * rs.ldr;
             const double *Bs = m.kd.get() + (size_t)s * S * m.kkey;
             uint32_t *Ds = sc.D.get() + (size_t)s * R * S;
             int rc;
-            if (mode == 2 && m.i8 && r0 % 32 == 0)
+            if (i8)
                 rc = contract_i8_tuned(rs.Adig.get() + s * rows_digit_limb_bytes(rs.cap, rs.ksteps) +
                                            (size_t)(r0 / 32) * rs.ksteps * 4 * 1024,
                                        rs.ksteps, m.kdig.get() + s * keys_digit_limb_bytes(m.spad, m.ksteps, m.nb),
-                                       m.ksteps, m.nb, R, S, s, Ds, st);
+                                       m.ksteps, m.nb, R, S, s, Ds, st, fuse_crt && s == 2 ? sc.D.get() : nullptr,
+                                       fuse_crt && s == 2 ? sc.D.get() + (size_t)R * S : nullptr,
+                                       fuse_crt && s == 2 ? sc.kval.get() : nullptr);
             else if (m.kb[s] < 1)
                 rc = launch_contract_slow(As, Bs, R, S, m.kkey, rs.ldr, m.kkey, s, dev->barrett, Ds, st);
             else if (mode >= 1 && r0 % 16 == 0)
@@ -386,7 +395,9 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
             if (tm && s < 2) HIP_OK(hipEventRecord(tm->ev[4 + s], st));
         }
         if (tm) HIP_OK(hipEventRecord(tm->ev[1], st));
-        if (m.fused) {
+        if (fuse_crt) {
+            // key values already written by limb 2's contraction
+        } else if (m.fused) {
             if (launch_crt_kvalue(sc.D.get(), (size_t)R * S, sc.kval.get(), st)) return -1;
         } else if (launch_kvalue(sc.D.get(), rs.c0.get() + (size_t)r0 * rs.l * 3, rs.l, m.y, R, S, sc.kval.get(),
                                  sc.flag.get(), st)) {

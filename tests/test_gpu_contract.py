@@ -1,6 +1,8 @@
 """The coefficient-0 contraction kernels agree bit for bit: the int8 matrix-core
-kernel in every block shape (and its autotuned choice), the fp64 scalar-operand
-kernel and the fp64 LDS kernel, on one model and one row store.  Keys too large
+kernel in every block shape (and its autotuned choice), with the CRT and
+rounding fused into the last limb's epilogue or run as a separate kernel, the
+fp64 scalar-operand kernel and the fp64 LDS kernel, on one model and one row
+store.  Keys too large
 for the two-digit split run on fp64 and still match the oracle."""
 import hashlib
 import os
@@ -67,7 +69,10 @@ def test_kernels_agree(gpu, hi, digits):
     variants = gpu.contract_variants(digits)
     note(f"variants {variants}")
     assert len(variants) >= 6
-    for env in [{"CUFE_I8_VARIANT": str(v)} for v in variants] + [{"CUFE_CONTRACT": "sgpr"},
+    # every variant (the fused CRT epilogue on limb 2), the separate CRT
+    # kernel, and the fp64 kernels
+    for env in [{"CUFE_I8_VARIANT": str(v)} for v in variants] + [{"CUFE_NO_FUSED_CRT": "1"},
+                                                                  {"CUFE_CONTRACT": "sgpr"},
                                                                   {"CUFE_CONTRACT": "lds"}]:
         note(f"kernel {env}")
         lab, kv = predict_with(ctx, R, env)

@@ -275,7 +275,7 @@ extern "C" int cufe_sample_polya(const unsigned char seed[32], uint32_t *a)
 
 extern "C" double cufe_round_extract2(const uint32_t *d_y)
 {
-    return (double)crt_round(crt_garner(d_y[0], d_y[IPFE_N], d_y[2 * IPFE_N]));
+    return (double)crt_decode(d_y[0], d_y[IPFE_N], d_y[2 * IPFE_N]);
 }
 
 // svm_predict_gpu (main.c:270-390), plain per-call form.
@@ -292,7 +292,7 @@ extern "C" double cufe_svm_predict_gpu(const cufe_svm_model *model,
         // round_extract_gmp2 reconstructs every coefficient and keeps only the first
         for (int k = 0; k < IPFE_N; ++k) {
             const uint32_t *dk = dy.data() + (size_t)i * kPoly + k;
-            uint32_t r = crt_round(crt_garner(dk[0], dk[IPFE_N], dk[2 * IPFE_N]));
+            uint32_t r = crt_decode(dk[0], dk[IPFE_N], dk[2 * IPFE_N]);
             if (k == 0) kv[i] = (double)r;
         }
     }

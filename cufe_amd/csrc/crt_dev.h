@@ -22,33 +22,28 @@ constexpr uint64_t CRT_C2 = 2935204199007202395ull % 2130706433ull;
 
 __host__ __device__ inline u128 crt_Q() { return (u128)CRT_Q01 * 2130706433ull; }
 
-__host__ __device__ inline u128 crt_garner(uint32_t v0, uint32_t v1, uint32_t v2)
+// round_extract(crt_reverse(v0, v1, v2)) for one coefficient, without
+// 128-bit division.  Garner's x = x01 + u2 Q01 (x01 = v0 + u1 q0 < Q01 +
+// 2^32 since v0 enters unreduced, u2 < q2, Q = Q01 q2).  With p u2 = A q2 + B
+// (B < q2):
+//     p x = A Q + N',   N' = B Q01 + p x01 < Q + p (Q01 + 2^32) < 2 Q,
+// so floor(p x / Q) = A + [N' >= Q], and rounding adds one when the
+// remainder exceeds floor(Q/2) (Q is odd: when twice the remainder exceeds Q):
+//     round(p x / Q) = A + [2 N' >= Q] + [2 N' >= 3 Q].
+__host__ __device__ inline uint32_t crt_decode(uint32_t v0, uint32_t v1, uint32_t v2)
 {
     const uint64_t q0 = qmod(0), q1 = qmod(1), q2 = qmod(2);
-    uint64_t x = v0;                       // < 2^25
+    uint64_t x = v0;
     uint64_t a1 = v1 % q1, x1 = x % q1;
     uint64_t u1 = ((a1 + q1 - x1) % q1) * CRT_C1 % q1;
-    x += u1 * q0;                          // < 2^56
+    x += u1 * q0;  // x01 < Q01 + 2^32 < 2^56
     uint64_t a2 = v2 % q2, x2 = x % q2;
-    uint64_t u2 = ((a2 + q2 - x2) % q2) * CRT_C2 % q2;
-    return (u128)x + (u128)u2 * CRT_Q01;
-}
-
-// round(p x / Q) as the reference computes it; x < 2^88.
-__host__ __device__ inline uint32_t crt_round(u128 x)
-{
+    const uint64_t u2 = ((a2 + q2 - x2) % q2) * CRT_C2 % q2;
+    const uint64_t pu = u2 * IPFE_P;  // < 2^47
+    const uint64_t A = pu / q2, B = pu - A * q2;
+    const u128 n2 = 2 * ((u128)B * CRT_Q01 + (u128)x * IPFE_P);  // 2 N' < 2^88
     const u128 Q = crt_Q();
-    const u128 a = x * (u128)IPFE_P;
-    // quotient estimate from doubles, then exact correction
-    double xd = (double)(uint64_t)(x >> 64) * 18446744073709551616.0 + (double)(uint64_t)x;
-    const double Qd = 76687145727357674227351553.0;
-    uint64_t quo = (uint64_t)(xd * (double)IPFE_P / Qd);
-    u128 prod = (u128)quo * Q;
-    while (prod > a) { --quo; prod -= Q; }
-    u128 rem = a - prod;
-    while (rem >= Q) { ++quo; rem -= Q; }
-    if (rem > Q / 2) ++quo;
-    return (uint32_t)quo;
+    return (uint32_t)(A + (n2 >= Q ? 1 : 0) + (n2 >= 3 * Q ? 1 : 0));
 }
 
 }  // namespace ipfe

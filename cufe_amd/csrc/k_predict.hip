@@ -357,7 +357,7 @@ __global__ void k_kvalue(const uint32_t *__restrict__ D, const uint32_t *__restr
         if (acc == q && dd == 0) atomicOr(flag, 1);
         d[s] = sub_q(acc, dd, q);
     }
-    kval[t] = crt_round(crt_garner(d[0], d[1], d[2]));
+    kval[t] = crt_decode(d[0], d[1], d[2]);
 }
 
 // kvalue[r][v] = round(p * CRT(D_0, D_1, D_2) / Q) when the contraction already
@@ -366,7 +366,7 @@ __global__ void k_crt_kvalue(const uint32_t *__restrict__ D, size_t n, uint32_t 
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    kval[t] = crt_round(crt_garner(D[t], D[n + t], D[2 * n + t]));
+    kval[t] = crt_decode(D[t], D[n + t], D[2 * n + t]);
 }
 
 // ---------------------------------------------------------------------------

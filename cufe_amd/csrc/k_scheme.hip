@@ -64,7 +64,7 @@ __global__ void k_crt_round(const uint32_t *__restrict__ d, int nvec, int coef, 
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvec) return;
     const uint32_t *dv = d + (size_t)v * 3 * IPFE_N + coef;
-    r[v] = crt_round(crt_garner(dv[0], dv[IPFE_N], dv[2 * IPFE_N]));
+    r[v] = crt_decode(dv[0], dv[IPFE_N], dv[2 * IPFE_N]);
 }
 
 int launch_keygen(const SparseY &y, int nvec, const uint32_t *msk, uint32_t *sk, hipStream_t st)

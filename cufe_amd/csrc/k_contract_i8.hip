@@ -75,7 +75,7 @@ __device__ __forceinline__ void i8_store(const v16i (&acc)[DA][NB], int row0, in
         }
         const size_t idx = (size_t)row * S + key;
         if (out.kval)
-            out.kval[idx] = crt_decode(out.d0[idx], out.d1[idx], (uint32_t)tot);
+            out.kval[idx] = crt_decode_canon(out.d0[idx], out.d1[idx], (uint32_t)tot);
         else
             out.D[idx] = (uint32_t)tot;
     }

@@ -366,7 +366,7 @@ __global__ void k_crt_kvalue(const uint32_t *__restrict__ D, size_t n, uint32_t 
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    kval[t] = crt_decode(D[t], D[n + t], D[2 * n + t]);
+    kval[t] = crt_decode_canon(D[t], D[n + t], D[2 * n + t]);  // canonical residues
 }
 
 // ---------------------------------------------------------------------------

@@ -2,8 +2,9 @@
 the host share), through its host-only entry point cufe_round_extract2,
 against the oracle and the closed form round(p x / Q) (src/crt.c:46-82,
 src/rlwe_sife.c:334-393): every residue edge (0, 1, q-1, q, q+1 in each limb,
-unreduced words), values beside each rounding boundary x = (2m+1) Q / (2p),
-and random residues.  No GPU: the call runs on the host."""
+unreduced words), values beside each rounding boundary x = (2m+1) Q / (2p)
+and around the distance where the fast double estimate hands over to exact
+128-bit arithmetic, and random residues.  No GPU: the call runs on the host."""
 import numpy as np
 import pytest
 
@@ -42,11 +43,16 @@ def test_edges_match_oracle(lib_host, po):
 
 
 def test_rounding_boundaries(lib_host):
+    """Beside each boundary, and at offsets around 7.6e11 where the library
+    switches from its double estimate to 128-bit integers (|2 N'/Q - 1| =
+    1e-9), on both sides."""
     rng = np.random.default_rng(41)
     for _ in range(3000):
         m = int(rng.integers(0, P))
         x0 = ((2 * m + 1) * QQ) // (2 * P)
-        for x in (x0 - 1, x0, x0 + 1, x0 + 2):
+        offsets = [-1, 0, 1, 2] + [int(s * 10 ** e) for s in (-1, 1) for e in (11.5, 11.9, 12.0, 13.0, 15.0)]
+        for dx in offsets:
+            x = min(max(x0 + dx, 0), QQ - 1)
             assert decode(lib_host, tuple(x % q for q in Q)) == closed_form(x), x
     # x just below Q rounds to p (no mod p), and 0 stays 0
     assert decode(lib_host, tuple((QQ - 1) % q for q in Q)) == P

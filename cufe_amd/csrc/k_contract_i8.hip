@@ -9,10 +9,11 @@
 //     sum_k a b = sum_e 256^e T_e,   T_e = sum_d 256^d P_de,
 //     P_de = sum_k a_d[k] b_e[k],
 // and each P_de is one int8 GEMM with exact int32 accumulation:
-// |P_de| <= K * 2^14 < 2^31 for K < 2^17.  The epilogue forms each T_e in 64
-// bits (|T_e| < 2^56), reduces it mod q, and combines the residues with the
-// weights 256^e < q, so D is the canonical residue of the exact integer dot
-// product -- the value the fp64 kernels produce.
+// |P_de| <= K * 2^14 < 2^31 for K < 2^17.  The epilogue forms each T_e
+// exactly in doubles (|T_e| < K 2^38 <= 2^52 for K <= 2^14), reduces it mod q, and
+// combines the residues with the weights 256^e < q, so D is the canonical
+// residue of the exact integer dot product -- the value the fp64 kernels
+// produce.
 //
 // Digit stores are laid out as MFMA fragments so every load is contiguous:
 //   rows, per limb: [row block of 32][k step of 32][digit d][lane][16 B]
@@ -46,38 +47,42 @@ struct I8Out {
 };
 
 // Epilogue shared by the kernels: lane (row0, key) holds acc[d][e][i] =
-// P_de for row row0 + 8 (i / 4) + i mod 4.  T_e = sum_d 256^d P_de (|T_e| <
-// 2^56) is reduced mod q, then the residues are combined with the weights
-// 256^e; the double quotients are off by at most one and corrected.
+// P_de for row row0 + 8 (i / 4) + i mod 4.  T_e = sum_d 256^d P_de is formed
+// in doubles by Horner steps, all exact: |P_de| <= K 2^14, so every partial
+// value stays below K 2^38 <= 2^52 (the launcher limits K to 2^14).  Each T_e
+// is reduced mod q (the double quotient is off by at most one; fma keeps the
+// remainder exact), then the residues are combined with the weights 256^e
+// (values below 2^40).
 template <int DA, int NB>
 __device__ __forceinline__ void i8_store(const v16i (&acc)[DA][NB], int row0, int key, int R, int S, uint32_t q,
                                          const I8Out &out)
 {
-    const double qinv = 1.0 / (double)q;
-    const long long qq = q;
+    const double qd = (double)q, qinv = 1.0 / qd;
+    auto reduce = [&](double t) {
+        double m = fma(-floor(t * qinv), qd, t);
+        m += (m < 0.0) ? qd : 0.0;
+        m -= (m >= qd) ? qd : 0.0;
+        return m;
+    };
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int row = row0 + (i >> 2) * 8 + (i & 3);
         if (row >= R) continue;
-        long long tot = 0;
+        double tot = 0.0;
 #pragma unroll
         for (int e = NB - 1; e >= 0; --e) {
-            long long T = 0;
+            double T = (double)acc[DA - 1][e][i];
 #pragma unroll
-            for (int d = 0; d < DA; ++d) T += (long long)acc[d][e][i] * (1ll << (8 * d));
-            long long m = T - (long long)floor((double)T * qinv) * qq;
-            m += (m < 0) ? qq : 0;
-            m -= (m >= qq) ? qq : 0;
-            tot = tot * 256 + m;
-            tot -= (long long)floor((double)tot * qinv) * qq;
-            tot += (tot < 0) ? qq : 0;
-            tot -= (tot >= qq) ? qq : 0;
+            for (int d = DA - 2; d >= 0; --d) T = fma(T, 256.0, (double)acc[d][e][i]);
+            const double m = reduce(T);
+            tot = (e == NB - 1) ? m : reduce(fma(tot, 256.0, m));
         }
+        const uint32_t res = (uint32_t)tot;
         const size_t idx = (size_t)row * S + key;
         if (out.kval)
-            out.kval[idx] = crt_decode_canon(out.d0[idx], out.d1[idx], (uint32_t)tot);
+            out.kval[idx] = crt_decode_canon(out.d0[idx], out.d1[idx], res);
         else
-            out.D[idx] = (uint32_t)tot;
+            out.D[idx] = res;
     }
 }
 
@@ -678,7 +683,7 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 {
 
     if (R <= 0 || S <= 0) return 0;
-    if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 17) || !i8_variant_ok(variant, nb, s)) {
+    if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 14) || !i8_variant_ok(variant, nb, s)) {
         ipfe_set_error("int8 contraction: %d k steps (row store %d), variant %d, %d key digits", ksteps, ksteps_a,
                        variant, nb);
         return -1;

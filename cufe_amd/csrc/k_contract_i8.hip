@@ -23,6 +23,9 @@
 // is D[8 (i / 4) + 4 (l / 32) + i mod 4][l mod 32].
 #include <stdio.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "crt_dev.h"
@@ -310,7 +313,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 // The main loop runs while the ring still has steps to load; the last
 // STAGES + 1 steps (at most) wait with the conservative vmcnt(0).
 // ---------------------------------------------------------------------------
-template <int WR, int WK, int NB, int STAGES, int DA>
+template <int WR, int WK, int NB, int STAGES, int DA, bool PT>
 __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *__restrict__ A,
                                                                     const v4i *__restrict__ B, int R, int S,
                                                                     int ksteps, long long a_stride,
@@ -327,32 +330,57 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
     __shared__ v4i ring[STAGES * STEP_V4];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int per = gridDim.x / 8;
-    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (t >= ntiles) return;
-    const int tk = t % ntk, tr = t / ntk;
     const int wr = w / WK, wk = w % WK;
-    const v4i *src[FW];
-    long long step[FW];
-#pragma unroll
-    for (int u = 0; u < FW; ++u) {
-        const int f = w + u * W;
-        if (f < WR * DG_A) {
-            src[u] = A + (size_t)(tr * WR + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4 + lane;
-            step[u] = DG_A * FRAG_V4;
-        } else {
-            const int fb = f - WR * DG_A;
-            src[u] = B + (size_t)(tk * WK + fb / NB) * b_stride + (fb % NB) * FRAG_V4 + lane;
-            step[u] = NB * FRAG_V4;
-        }
+    // This block's tiles t0, t0 + tstride, ... (nt of them).  The tiles are
+    // split into 8 contiguous ranges, one per XCD (block b runs on XCD b mod
+    // 8), so the blocks of one XCD work on neighbouring tiles -- the key tiles
+    // of the same row tiles -- and share the row digits in that XCD's L2.
+    int t0, tstride, nt;
+    if (PT) {
+        const int bx = gridDim.x / 8, per = (ntiles + 7) / 8;
+        const int lo = (int)(blockIdx.x % 8) * per, hi = min(ntiles, lo + per);
+        t0 = lo + (int)(blockIdx.x / 8);
+        tstride = bx;
+        nt = t0 < hi ? (hi - t0 + bx - 1) / bx : 0;
+    } else {
+        t0 = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+        tstride = 0;
+        nt = t0 < ntiles ? 1 : 0;
     }
-    auto issue = [&](int ks) {
-        v4i *slot = ring + (ks % STAGES) * STEP_V4;
+    if (nt == 0) return;
+    // the block's k steps in order: step g = tile g / ksteps, k step g mod ksteps
+    const int G = nt * ksteps;
+    long long step[FW];
+    const v4i *base[FW];
+    auto set_base = [&](int j) {
+        const int t = t0 + j * tstride, tk = t % ntk, tr = t / ntk;
 #pragma unroll
         for (int u = 0; u < FW; ++u) {
             const int f = w + u * W;
-            __builtin_amdgcn_global_load_lds((glb_void_t *)(src[u] + (size_t)ks * step[u]),
+            if (f < WR * DG_A) {
+                base[u] = A + (size_t)(tr * WR + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4 + lane;
+                step[u] = DG_A * FRAG_V4;
+            } else {
+                const int fb = f - WR * DG_A;
+                base[u] = B + (size_t)(tk * WK + fb / NB) * b_stride + (fb % NB) * FRAG_V4 + lane;
+                step[u] = NB * FRAG_V4;
+            }
+        }
+    };
+    set_base(0);
+    int it_j = 0, it_ks = 0, it_slot = 0;  // next step to load
+    auto issue_next = [&]() {
+        v4i *slot = ring + it_slot * STEP_V4;
+#pragma unroll
+        for (int u = 0; u < FW; ++u) {
+            const int f = w + u * W;
+            __builtin_amdgcn_global_load_lds((glb_void_t *)(base[u] + (size_t)it_ks * step[u]),
                                              (lds_void_t *)(slot + f * FRAG_V4), 16, 0, 0);
+        }
+        it_slot = it_slot + 1 == STAGES ? 0 : it_slot + 1;
+        if (++it_ks == ksteps) {
+            it_ks = 0;
+            if (++it_j < nt) set_base(it_j);
         }
     };
     const int a_off = wr * DG_A * FRAG_V4 + lane;
@@ -364,8 +392,8 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
         for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
 #pragma unroll
     for (int p = 0; p < STAGES; ++p)
-        if (p < ksteps) issue(p);
-    if (ksteps >= STAGES)
+        if (p < G) issue_next();
+    if (G >= STAGES)
         wait_vmcnt<(STAGES - 1) * FW>();
     else
         wait_vmcnt<0>();
@@ -375,12 +403,13 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
     for (int d = 0; d < DA; ++d) fa0[d] = ring[a_off + d * FRAG_V4];
 #pragma unroll
     for (int e = 0; e < NB; ++e) fb0[e] = ring[b_off + e * FRAG_V4];
-    // One step.  In the last step the reads of "step ksteps" fetch a slot no
-    // one loads any more: in bounds, and never used.  Every wave runs the same
+    int rd_slot = 1 % STAGES;  // slot of step g + 1
+    // One step g.  In the last step the reads of "step G" fetch a slot no one
+    // loads any more: in bounds, and never used.  Every wave runs the same
     // number of barriers.
-    auto body = [&](auto main, int ks, v4i (&fa)[DA], v4i (&fb)[NB], v4i (&na)[DA], v4i (&nb)[NB]) {
+    auto body = [&](auto main, int g, v4i (&fa)[DA], v4i (&fb)[NB], v4i (&na)[DA], v4i (&nb)[NB]) {
         constexpr bool M = decltype(main)::value;
-        const v4i *nx = ring + ((ks + 1) % STAGES) * STEP_V4;
+        const v4i *nx = ring + rd_slot * STEP_V4;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
             acc[i / NB][i % NB] =
@@ -394,9 +423,9 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
                 asm volatile("s_barrier" ::: "memory");
             } else if (i == 2) {
                 if constexpr (M)
-                    issue(ks + STAGES);
-                else if (ks + STAGES < ksteps)
-                    issue(ks + STAGES);
+                    issue_next();
+                else if (g + STAGES < G)
+                    issue_next();
             } else if (i == 3) {
 #pragma unroll
                 for (int e = 0; e < NB; ++e) nb[e] = nx[b_off + e * FRAG_V4];
@@ -408,23 +437,47 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
             }
             __builtin_amdgcn_sched_barrier(0);
         }
+        rd_slot = rd_slot + 1 == STAGES ? 0 : rd_slot + 1;
+    };
+    auto store = [&](int j) {
+        const int t = t0 + j * tstride, tk = t % ntk, tr = t / ntk;
+        const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
+        if (key < S) i8_store<DA, NB>(acc, tr * (32 * WR) + wr * 32 + (lane >> 5) * 4, key, R, S, q, out);
+    };
+    // after step g of a persistent block: at the end of a tile, store it and
+    // restart the sums (a one-tile block stores after its loop)
+    int c_j = 0, c_ks = 0;
+    auto tile_end = [&]() {
+        if constexpr (PT) {
+            if (++c_ks < ksteps) return;
+            store(c_j);
+#pragma unroll
+            for (int d = 0; d < DA; ++d)
+#pragma unroll
+                for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+            c_ks = 0;
+            ++c_j;
+        }
     };
     using Main = std::integral_constant<bool, true>;
     using Tail = std::integral_constant<bool, false>;
-    int ks = 0;
-    // main: steps ks and ks + 1 both still load step + STAGES
-    for (; ks + 1 + STAGES < ksteps; ks += 2) {
-        body(Main{}, ks, fa0, fb0, fa1, fb1);
-        body(Main{}, ks + 1, fa1, fb1, fa0, fb0);
+    int g = 0;
+    // main: steps g and g + 1 both still load step + STAGES
+    for (; g + 1 + STAGES < G; g += 2) {
+        body(Main{}, g, fa0, fb0, fa1, fb1);
+        tile_end();
+        body(Main{}, g + 1, fa1, fb1, fa0, fb0);
+        tile_end();
     }
-    for (; ks < ksteps; ks += 2) {
-        body(Tail{}, ks, fa0, fb0, fa1, fb1);
-        if (ks + 1 < ksteps) body(Tail{}, ks + 1, fa1, fb1, fa0, fb0);
+    for (; g < G; g += 2) {
+        body(Tail{}, g, fa0, fb0, fa1, fb1);
+        tile_end();
+        if (g + 1 < G) {
+            body(Tail{}, g + 1, fa1, fb1, fa0, fb0);
+            tile_end();
+        }
     }
-    const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
-    if (key >= S) return;
-    const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
-    i8_store<DA, NB>(acc, row0, key, R, S, q, out);
+    if constexpr (!PT) store(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -612,7 +665,7 @@ int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int
 // Kernel variants the launcher can run: row blocks x key blocks per block
 // tile, the LDS ring depth (0: the register-staged double-buffered kernel),
 // the kernel, and the row digits the MFMAs use (3: limb 0 only).
-enum I8Kind { I8_REG, I8_RING, I8_RING_K2, I8_HYBRID, I8_INTERLEAVED };
+enum I8Kind { I8_REG, I8_RING, I8_RING_K2, I8_HYBRID, I8_INTERLEAVED, I8_PERSISTENT };
 struct I8Variant {
     int wr, wk, stages, kind, da;
 };
@@ -624,6 +677,8 @@ static const I8Variant kVariants[] = {
     {2, 2, 4, I8_INTERLEAVED, 4}, {1, 4, 4, I8_INTERLEAVED, 4}, {2, 4, 4, I8_INTERLEAVED, 4},
     {2, 2, 3, I8_INTERLEAVED, 3}, {2, 2, 4, I8_INTERLEAVED, 3}, {1, 4, 4, I8_INTERLEAVED, 3},
     {2, 4, 4, I8_INTERLEAVED, 3}, {1, 4, 3, I8_INTERLEAVED, 4}, {1, 4, 3, I8_INTERLEAVED, 3},
+    {2, 2, 4, I8_PERSISTENT, 4},  {2, 4, 4, I8_PERSISTENT, 4},  {2, 2, 4, I8_PERSISTENT, 3},
+    {2, 4, 4, I8_PERSISTENT, 3},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -635,8 +690,8 @@ void i8_shape(int v, int *wr, int *wk)
 }
 int i8_variant_stages(int v) { return kVariants[v].stages; }
 // "WRxWK" for the register-staged kernel; "WRxWK/STAGES" plus "k2" (two k
-// steps per barrier), "h" (hybrid) or "i" (interleaved; "i3": three row
-// digits) for the others
+// steps per barrier), "h" (hybrid), "i" (interleaved; "i3": three row
+// digits) or "ip" (interleaved, persistent; "ip3") for the others
 const char *i8_variant_label(int v)
 {
     struct Labels {
@@ -648,6 +703,7 @@ const char *i8_variant_label(int v)
                 const char *suffix = x.kind == I8_RING_K2       ? "k2"
                                      : x.kind == I8_HYBRID      ? "h"
                                      : x.kind == I8_INTERLEAVED ? (x.da == 3 ? "i3" : "i")
+                                     : x.kind == I8_PERSISTENT  ? (x.da == 3 ? "ip3" : "ip")
                                                                 : "";
                 if (x.kind == I8_REG)
                     snprintf(s[i], sizeof s[i], "%dx%d", x.wr, x.wk);
@@ -670,7 +726,30 @@ bool i8_variant_ok(int v, int nb, int s)
     if (x.da == 3 && s != 0) return false;
     if (nb == 2 || x.kind == I8_REG) return true;
     if (x.kind == I8_HYBRID) return x.stages == 3;
-    return x.wr == 1 && x.kind != I8_INTERLEAVED;
+    return x.wr == 1 && x.kind != I8_INTERLEAVED && x.kind != I8_PERSISTENT;
+}
+
+// Grid of a persistent kernel: every CU filled once at the kernel's
+// occupancy, a multiple of 8 (one share per XCD), no more than the tiles.
+static int persistent_grid(const void *fn, int block_threads, int ntiles)
+{
+    static std::mutex mu;
+    static std::map<const void *, int> per_cu;
+    static int cus = 0;
+    int nb = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!cus && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) cus = 256;
+        auto it = per_cu.find(fn);
+        if (it == per_cu.end()) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block_threads, 0) != hipSuccess || nb < 1) nb = 1;
+            per_cu[fn] = nb;
+        } else {
+            nb = it->second;
+        }
+    }
+    const int tiles8 = (ntiles + 7) / 8 * 8;
+    return std::max(8, std::min(nb * cus, tiles8) / 8 * 8);
 }
 
 // A: row digit store of one limb at row r0 (r0 a multiple of 32; the store
@@ -714,8 +793,13 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
     hipLaunchKernelGGL((k_contract_i8_dma<wr, wk, n, stg, kpb>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, \
                        ntk, ntiles)
 #define CI8I(wr, wk, n, stg, da)                                                                                     \
-    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D,  \
-                       ntk, ntiles)
+    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, false>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, \
+                       q, D, ntk, ntiles)
+#define CI8P(wr, wk, n, stg, da)                                                                                     \
+    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, true>),                                                 \
+                       dim3(persistent_grid(reinterpret_cast<const void *>(&k_contract_i8_il<wr, wk, n, stg, da, true>), \
+                                            block.x, ntiles)),                                                      \
+                       block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
 #define CI8H(wk, n, stg)                                                                                             \
     hipLaunchKernelGGL((k_contract_i8_hyb<wk, n, stg>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk,    \
                        ntiles)
@@ -741,7 +825,11 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         case 17: CI8I(1, 4, 2, 4, 3); break;
         case 18: CI8I(2, 4, 2, 4, 3); break;
         case 19: CI8I(1, 4, 2, 3, 4); break;
-        default: CI8I(1, 4, 2, 3, 3); break;  // variant 20
+        case 20: CI8I(1, 4, 2, 3, 3); break;
+        case 21: CI8P(2, 2, 2, 4, 4); break;
+        case 22: CI8P(2, 4, 2, 4, 4); break;
+        case 23: CI8P(2, 2, 2, 4, 3); break;
+        default: CI8P(2, 4, 2, 4, 3); break;  // variant 24
         }
     } else {
         switch (variant) {
@@ -758,6 +846,7 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 #undef CI8D
 #undef CI8H
 #undef CI8I
+#undef CI8P
     (void)ST;
     HIP_OK(hipGetLastError());
     return 0;

@@ -50,6 +50,7 @@ def lib() -> C.CDLL:
         "cufe_ntt_forward": (C.c_int, [_u32p, C.c_int]),
         "cufe_ntt_inverse": (C.c_int, [_u32p, C.c_int]),
         "cufe_gauss_sample": (C.c_int, [_u8p, _u16p, C.c_int, C.c_int, C.c_int, _u32p]),
+        "cufe_gauss_sample_first": (C.c_int, [_u8p, _u16p, C.c_int, C.c_int, C.c_int, _u32p]),
         "cufe_aes256ctr": (C.c_int, [_u8p, C.c_uint16, C.c_uint32, C.c_uint32, _u8p]),
         "cufe_crt_round": (C.c_int, [_u32p, C.c_int, C.c_int, _u32p]),
         "cufe_sample_polya": (C.c_int, [_u8p, _u32p]),
@@ -135,6 +136,16 @@ def gauss_sample(keys, nonces, which: int, ncalls: int) -> np.ndarray:
     n = np.ascontiguousarray(nonces, dtype=np.uint16)
     out = np.empty((len(n), ncalls, NMODULI, N), dtype=np.uint32)
     _check(lib().cufe_gauss_sample(k, n, len(n), which, ncalls, out), "gauss_sample")
+    return out
+
+
+def gauss_sample_first(keys, nonces, which: int, ncalls: int) -> np.ndarray:
+    """Sample 0 of each of ncalls consecutive calls on each stream, sampled
+    the way the resident row encryption samples its f_i -> [nstreams][ncalls][3]."""
+    k = np.ascontiguousarray(np.frombuffer(b"".join(bytes(x) for x in keys), dtype=np.uint8))
+    n = np.ascontiguousarray(nonces, dtype=np.uint16)
+    out = np.empty((len(n), ncalls, NMODULI), dtype=np.uint32)
+    _check(lib().cufe_gauss_sample_first(k, n, len(n), which, ncalls, out), "gauss_sample_first")
     return out
 
 

@@ -385,6 +385,30 @@ extern "C" int cufe_gauss_sample(const unsigned char *keys, const uint16_t *nonc
     return sync_stream(st);
 }
 
+extern "C" int cufe_gauss_sample_first(const unsigned char *keys, const uint16_t *nonces, int nstreams, int which,
+                                       int ncalls, uint32_t *out)
+{
+    Device *dev = device();
+    if (!dev) return -1;
+    if (which < 1 || which > 3 || ncalls < 0 || nstreams < 0) {
+        ipfe_set_error("cufe_gauss_sample_first: bad arguments (which=%d ncalls=%d nstreams=%d)", which, ncalls,
+                       nstreams);
+        return -1;
+    }
+    if (ncalls == 0 || nstreams == 0) return 0;
+    hipStream_t st = dev->stream;
+    DevBuf<uint32_t> d;
+    const size_t per_stream = (size_t)ncalls * 3;
+    if (d.alloc((size_t)nstreams * per_stream)) return -1;
+    std::vector<StreamReq> reqs;
+    for (int t = 0; t < nstreams; ++t)
+        reqs.push_back(make_stream(keys + 32 * (size_t)t, nonces[t], which, ncalls, d.get() + (size_t)t * per_stream,
+                                   nullptr, 0, 1));
+    if (run_gauss(reqs, st)) return -1;
+    HIP_OK(hipMemcpyAsync(out, d.get(), (size_t)nstreams * per_stream * 4, hipMemcpyDeviceToHost, st));
+    return sync_stream(st);
+}
+
 extern "C" int cufe_aes256ctr(const unsigned char key[32], uint16_t nonce, uint32_t ctr0, uint32_t nblocks,
                               unsigned char *out)
 {

@@ -110,10 +110,20 @@ struct Cand {
 constexpr int PART_BLOCKS = 17;
 __device__ __forceinline__ int chunk_block(int j, int full) { return (full || j < 16) ? j : 17; }
 
+// Sign word (bytes 280..283) read from the chunk in LDS.
+struct SignInLds {
+    __device__ uint32_t operator()(const uint32_t *ch) const { return ch[70]; }
+};
+
 // ulim: uniform words a partial chunk may scan (14 covers its blocks; lower
-// values exist to exercise the rerun)
+// values exist to exercise the rerun).  sign_word(ch) returns bytes 280..283
+// of the chunk, which hold the candidate's sign bit; with sign_if_zero it is
+// asked only for a zero candidate (the only one whose acceptance depends on
+// its sign), and the sign of every other candidate is reported as 0.
+template <typename SignWord = SignInLds>
 __device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int which, bool valid, int *status,
-                                               int full, int ulim)
+                                               int full, int ulim, SignWord sign_word = SignWord(),
+                                               bool sign_if_zero = false)
 {
     const uint64_t K = c_K[which], BF = c_BF[which];
     const uint32_t UQ = (uint32_t)c_UQ[which];
@@ -163,18 +173,20 @@ __device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int wh
     out.z = xk + y;
     const uint64_t bin = (uint64_t)(uint32_t)(out.z + xk) * (uint64_t)(uint32_t)y;
     const bool bit = bernoulli_bit(bin, K2INV, gb + 16, ln);
-    out.sgn = (ch[70] >> c) & 1u;
+    out.sgn = (!sign_if_zero || out.z == 0) ? (sign_word(ch) >> c) & 1u : 0u;
     out.acc = bit && (out.z != 0 || out.sgn != 0);
     return out;
 }
 
 // AES of nch chunks of one stream into LDS: chunk k at ks + 96 k uses
 // counters 24 (chunk_of(k)) .. +23.
+// nb: blocks per chunk -- 24 (all), PART_BLOCKS (0..15 and 17) or 16 (0..15,
+// for streams whose values are not stored: the sign byte then matters only for
+// a zero candidate, and that block is computed where it is needed).
 template <bool REP, int NT, typename ChunkOf>
 __device__ __forceinline__ void aes_chunks(const GaussStream *S, const uint32_t *te, int nch, ChunkOf chunk_of,
-                                           uint32_t *ks, int full)
+                                           uint32_t *ks, int full, int nb)
 {
-    const int nb = full ? 24 : PART_BLOCKS;
     for (int g = threadIdx.x; g < nch * nb; g += NT) {
         const int k = g / nb, blk = chunk_block(g - nb * k, full);
         const uint32_t ctr = (uint32_t)(chunk_of(k) * 24 + blk);
@@ -213,13 +225,30 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
         const int nch = (int)min((long long)TILE_CHUNKS, (long long)S->nchunks - chunk0);
         const int which = S->which;
         const bool store_vals = !S->coef0_only;
+        // without stored values only a zero candidate needs its sign (a zero
+        // with sign 0 is rejected, src/gauss.c:1086-1163)
+        const bool lazy_sign = !store_vals && !full;
         __syncthreads();  // tables staged / previous tile's reads of ks done
-        aes_chunks<REP, NT>(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks, full);
+        aes_chunks<REP, NT>(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks, full,
+                            full ? 24 : lazy_sign ? 16 : PART_BLOCKS);
         __syncthreads();
         for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += NT) {
             const int k = slot >> 3, c = slot & 7;
             const bool valid = k < nch;
-            const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status, full, ulim);
+            // with lazy_sign, block 17 of chunk k is computed by the asking lane
+            // alone (a zero candidate: about 1e-9 of S2/S3 candidates)
+            auto sign_word = [&](const uint32_t *ch) -> uint32_t {
+                if (!lazy_sign) return ch[70];
+                uint32_t o[4];
+                const uint32_t ctr = (uint32_t)((chunk0 + k) * 24 + 17);
+                if (REP)
+                    aes256_ctr_block_rep(S->rk, te, S->nonce, ctr, o);
+                else
+                    aes256_ctr_block(S->rk, te, S->nonce, ctr, o);
+                return o[2];
+            };
+            const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status, full, ulim, sign_word,
+                                           lazy_sign);
             const uint64_t am = __ballot(cd.acc);
             const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
             if (valid) {

@@ -113,6 +113,12 @@ int cufe_ntt_inverse(uint32_t *polys, int npoly);  /* GS_reverse incl. N^-1 */
  * out: [nstreams][ncalls][3][4096]. */
 int cufe_gauss_sample(const unsigned char *keys, const uint16_t *nonces, int nstreams, int which, int ncalls,
                       uint32_t *out);
+/* The same streams sampled the way the resident row encryption samples its
+ * f_i: only sample 0 of every call is produced (the stream is still walked
+ * in full).  out: [nstreams][ncalls][3], equal to cufe_gauss_sample's
+ * coefficient 0. */
+int cufe_gauss_sample_first(const unsigned char *keys, const uint16_t *nonces, int nstreams, int which, int ncalls,
+                            uint32_t *out);
 int cufe_aes256ctr(const unsigned char key[32], uint16_t nonce, uint32_t ctr0, uint32_t nblocks, unsigned char *out);
 /* CRT + rounding of coefficient coef of nvec RNS polys d[nvec][3][4096]. */
 int cufe_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r);

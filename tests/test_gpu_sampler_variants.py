@@ -2,7 +2,8 @@
 bank-replicated and the four-table T-table AES, and a lowered partial-chunk
 scan limit that makes many chunks ask for all 24 AES blocks (the rerun path
 that natural data reaches about once in 1e11 chunks).  Checked on multi-call
-streams (full outputs) and on compact row encryption (coefficient 0 only)."""
+streams (full outputs), on compact streams (sample 0 of each call, against the
+full streams) and on compact row encryption (coefficient 0 only)."""
 import hashlib
 import os
 
@@ -48,6 +49,24 @@ def test_stream_variants_match_oracle(gpu, po, which, variant):
         got = gpu.gauss_sample(keys, [5, 6, 7], which, 2)
     for t in range(3):
         assert np.array_equal(got[t], po.gauss_calls(keys[t], 5 + t, which, 2)), (t, variant)
+
+
+@pytest.mark.parametrize("variant", [{}, {"CUFE_AES_REP": "0"}, {"CUFE_GAUSS_SCAN": "8"}])
+@pytest.mark.parametrize("which", [1, 2, 3])
+def test_first_samples_equal_full_streams(gpu, which, variant):
+    """Compact streams compute the sign block of a chunk only for a zero
+    candidate (whose acceptance depends on its sign).  With S1 (k = 266) zero
+    candidates are common -- about 0.2% of candidates, some 20,000 here --
+    so the call boundaries, and with them every first sample, depend on that
+    path being right."""
+    note(f"first samples S{which} {variant}")
+    nst, ncalls = 600, 3
+    keys = [h(b"f%d-%d" % (which, t)) for t in range(nst)]
+    nonces = list(range(nst))
+    with env(variant):
+        first = gpu.gauss_sample_first(keys, nonces, which, ncalls)
+        full = gpu.gauss_sample(keys, nonces, which, ncalls)
+    assert np.array_equal(first, full[:, :, :, 0])
 
 
 def test_compact_rows_variants_agree(gpu):

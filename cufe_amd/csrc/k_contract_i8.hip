@@ -760,14 +760,13 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
                        uint32_t *D_out, hipStream_t st, int variant, const uint32_t *crt_d0, const uint32_t *crt_d1,
                        uint32_t *kval)
 {
-
     if (R <= 0 || S <= 0) return 0;
     if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 14) || !i8_variant_ok(variant, nb, s)) {
         ipfe_set_error("int8 contraction: %d k steps (row store %d), variant %d, %d key digits", ksteps, ksteps_a,
                        variant, nb);
         return -1;
     }
-    const int WR = kVariants[variant].wr, WK = kVariants[variant].wk, ST = kVariants[variant].stages;
+    const int WR = kVariants[variant].wr, WK = kVariants[variant].wk;
     // row tiles never reach past the 128 padding rows; key tiles stay within
     // the 128-key padding
     const int key_tile = 32 * WK, row_tile = 32 * WR;
@@ -847,7 +846,6 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 #undef CI8H
 #undef CI8I
 #undef CI8P
-    (void)ST;
     HIP_OK(hipGetLastError());
     return 0;
 }

@@ -69,17 +69,20 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
         if (it != g_tune.end())
             return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, it->second, crt_d0, crt_d1, kval);
     }
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
+    // events released on every path out
+    struct EventPair {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~EventPair()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    HIP_OK(hipEventCreate(&ev.e[0]));
+    HIP_OK(hipEventCreate(&ev.e[1]));
     // one untimed launch loads the code objects
     if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, 0, crt_d0, crt_d1, kval)) return -1;
     std::vector<float> times(i8_shape_count(), -1.0f);
-    int best = 0;
-    float best_ms = 1e30f;
-    char line[512];
-    int off = snprintf(line, sizeof line, "S=%d R=%d ksteps=%d key digits=%d limb%s:", S, R, ksteps, nb,
-                       s == 0 ? " 0" : "s 1-2");
     // Two passes over the variants, the second in reverse order, and the
     // faster launch of each counts: the clock drifts as the GPU warms during
     // the sweep, and one slow launch (a clock change, another stream's work)
@@ -90,17 +93,24 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
     for (int pass = 0; pass < 2; ++pass) {
         for (size_t n = 0; n < valid.size(); ++n) {
             const int v = pass == 0 ? valid[n] : valid[valid.size() - 1 - n];
-            HIP_OK(hipEventRecord(e0, st));
+            HIP_OK(hipEventRecord(ev.e[0], st));
             if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v, crt_d0, crt_d1, kval)) return -1;
-            HIP_OK(hipEventRecord(e1, st));
-            HIP_OK(hipEventSynchronize(e1));
+            HIP_OK(hipEventRecord(ev.e[1], st));
+            HIP_OK(hipEventSynchronize(ev.e[1]));
             float t = 0;
-            HIP_OK(hipEventElapsedTime(&t, e0, e1));
+            HIP_OK(hipEventElapsedTime(&t, ev.e[0], ev.e[1]));
             if (times[v] < 0 || t < times[v]) times[v] = t;
         }
     }
+    char buf[96];
+    snprintf(buf, sizeof buf, "S=%d R=%d ksteps=%d key digits=%d limb%s:", S, R, ksteps, nb,
+             s == 0 ? " 0" : "s 1-2");
+    std::string line = buf;
+    int best = valid.empty() ? 0 : valid[0];
+    float best_ms = 1e30f;
     for (int v : valid) {
-        off += snprintf(line + off, sizeof line - off, " %s %.3f", i8_variant_label(v), times[v]);
+        snprintf(buf, sizeof buf, " %s %.3f", i8_variant_label(v), times[v]);
+        line += buf;
         if (times[v] < best_ms) {
             best_ms = times[v];
             best = v;
@@ -108,18 +118,15 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
     }
     // Ties within 2% go to the variant with the most rows per block tile: it
     // re-reads the key operand from beyond L2 the fewest times.
-    for (int v = 0; v < i8_shape_count(); ++v) {
-        if (times[v] < 0 || times[v] > 1.02f * best_ms) continue;
+    for (int v : valid) {
+        if (times[v] > 1.02f * best_ms) continue;
         int wr, wk, bwr, bwk;
         i8_shape(v, &wr, &wk);
         i8_shape(best, &bwr, &bwk);
         if (wr > bwr || (wr == bwr && times[v] < times[best])) best = v;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    int wr, wk;
-    i8_shape(best, &wr, &wk);
-    snprintf(line + off, sizeof line - off, " ms -> variant %d (%s)\n", best, i8_variant_label(best));
+    snprintf(buf, sizeof buf, " ms -> variant %d (%s)\n", best, i8_variant_label(best));
+    line += buf;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = best;
     g_tune_log += line;

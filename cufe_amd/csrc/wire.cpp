@@ -78,7 +78,7 @@ size_t payload_words(int kind, int count, int l)
 }
 
 // limb of payload word w for each layout (to check residues against q)
-int limb_of(int kind, int l, size_t w, int count)
+int limb_of(int kind, size_t w, int count)
 {
     switch (kind) {
     case CUFE_WIRE_ROWS: {
@@ -117,7 +117,7 @@ extern "C" int cufe_wire_pack(int kind, const uint32_t *data, const uint32_t *da
     const size_t first = kind == CUFE_WIRE_ROWS ? (size_t)count * kPoly : words;
     for (size_t w = 0; w < words; ++w) {
         const uint32_t v = w < first ? data[w] : data2[w - first];
-        const uint32_t q = qmod(limb_of(kind, l, w, count));
+        const uint32_t q = qmod(limb_of(kind, w, count));
         if (v > q) {
             ipfe_set_error("cufe_wire_pack: word %zu = %u is not a residue mod %u", w, v, q);
             return -1;
@@ -181,7 +181,7 @@ extern "C" int cufe_wire_parse(const void *blob, size_t len, cufe_wire_info *inf
     }
     for (size_t w = 0; w < words; ++w) {
         const uint32_t v = get32(pay + 4 * w);
-        if (v > qmod(limb_of(kind, (int)l, w, (int)count))) {
+        if (v > qmod(limb_of(kind, w, (int)count))) {
             ipfe_set_error("cufe_wire_parse: word %zu = %u is not a residue", w, v);
             return -1;
         }

@@ -481,6 +481,159 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
 }
 
 // ---------------------------------------------------------------------------
+// Split-K form for small row counts (serving batches): a few row tiles leave
+// most CUs idle and each launch waits for its whole k loop, so the three limbs
+// and nsplit slices of the k steps run as separate blocks of one launch --
+// block (limb, slice, tile) -- with the interleaved pipeline of the kernel
+// above over its slice.  Each block writes the residue of its partial sum,
+// Dpart[limb][slice][R][S]; k_split_finish adds the slices mod q and does the
+// CRT and rounding.  Two key digits, four row digits on every limb.
+// ---------------------------------------------------------------------------
+template <int WR, int WK, int STAGES>
+__global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_split(
+    const v4i *__restrict__ A, long long a_limb, const v4i *__restrict__ B, long long b_limb, int R, int S, int ksteps,
+    long long a_stride, long long b_stride, int nsplit, int kper, int ntk, int ntiles, uint32_t *__restrict__ Dpart)
+{
+    constexpr int NB = 2, DA = DG_A;
+    constexpr int W = WR * WK;
+    constexpr int F = WR * DG_A + WK * NB;
+    static_assert(F % W == 0, "fragments must split evenly over the waves");
+    constexpr int FW = F / W;
+    constexpr int STEP_V4 = F * FRAG_V4;
+    constexpr int NM = DA * NB;
+    __shared__ v4i ring[STAGES * STEP_V4];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wr = w / WK, wk = w % WK;
+    const int t = (int)(blockIdx.x % ntiles), rest = (int)(blockIdx.x / ntiles);
+    const int slice = rest % nsplit, s = rest / nsplit;
+    const int k0 = slice * kper, G = min(kper, ksteps - k0);
+    if (G <= 0) return;  // whole block
+    const int tk = t % ntk, tr = t / ntk;
+    const v4i *Al = A + s * a_limb, *Bl = B + s * b_limb;
+    const v4i *base[FW];
+    long long step[FW];
+#pragma unroll
+    for (int u = 0; u < FW; ++u) {
+        const int f = w + u * W;
+        if (f < WR * DG_A) {
+            base[u] = Al + (size_t)(tr * WR + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4 + lane;
+            step[u] = DG_A * FRAG_V4;
+        } else {
+            const int fb = f - WR * DG_A;
+            base[u] = Bl + (size_t)(tk * WK + fb / NB) * b_stride + (fb % NB) * FRAG_V4 + lane;
+            step[u] = NB * FRAG_V4;
+        }
+    }
+    int it_ks = k0, it_slot = 0;
+    auto issue_next = [&]() {
+        v4i *slot = ring + it_slot * STEP_V4;
+#pragma unroll
+        for (int u = 0; u < FW; ++u) {
+            const int f = w + u * W;
+            __builtin_amdgcn_global_load_lds((glb_void_t *)(base[u] + (size_t)it_ks * step[u]),
+                                             (lds_void_t *)(slot + f * FRAG_V4), 16, 0, 0);
+        }
+        it_slot = it_slot + 1 == STAGES ? 0 : it_slot + 1;
+        ++it_ks;
+    };
+    const int a_off = wr * DG_A * FRAG_V4 + lane;
+    const int b_off = (WR * DG_A + wk * NB) * FRAG_V4 + lane;
+    v16i acc[DA][NB];
+#pragma unroll
+    for (int d = 0; d < DA; ++d)
+#pragma unroll
+        for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+#pragma unroll
+    for (int p = 0; p < STAGES; ++p)
+        if (p < G) issue_next();
+    if (G >= STAGES)
+        wait_vmcnt<(STAGES - 1) * FW>();
+    else
+        wait_vmcnt<0>();
+    asm volatile("s_barrier" ::: "memory");
+    v4i fa0[DA], fb0[NB], fa1[DA], fb1[NB];
+#pragma unroll
+    for (int d = 0; d < DA; ++d) fa0[d] = ring[a_off + d * FRAG_V4];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) fb0[e] = ring[b_off + e * FRAG_V4];
+    int rd_slot = 1 % STAGES;
+    // as in k_contract_i8_il: the reads of "step G" in the last step fetch a
+    // slot no one loads any more (in bounds, unused)
+    auto body = [&](auto main, int g, v4i (&fa)[DA], v4i (&fb)[NB], v4i (&na)[DA], v4i (&nb)[NB]) {
+        constexpr bool M = decltype(main)::value;
+        const v4i *nx = ring + rd_slot * STEP_V4;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            acc[i / NB][i % NB] =
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i / NB], fb[i % NB], acc[i / NB][i % NB], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (i == 1) {
+                if constexpr (M)
+                    wait_vmcnt<(STAGES - 2) * FW>();
+                else
+                    wait_vmcnt<0>();
+                asm volatile("s_barrier" ::: "memory");
+            } else if (i == 2) {
+                if constexpr (M)
+                    issue_next();
+                else if (g + STAGES < G)
+                    issue_next();
+            } else if (i == 3) {
+#pragma unroll
+                for (int e = 0; e < NB; ++e) nb[e] = nx[b_off + e * FRAG_V4];
+            } else if (i >= 4) {
+#pragma unroll
+                for (int d = 0; d < DA; ++d)
+                    if (d == i - 4 || (i == NM - 1 && d > i - 4)) na[d] = nx[a_off + d * FRAG_V4];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        rd_slot = rd_slot + 1 == STAGES ? 0 : rd_slot + 1;
+    };
+    using Main = std::integral_constant<bool, true>;
+    using Tail = std::integral_constant<bool, false>;
+    int g = 0;
+    for (; g + 1 + STAGES < G; g += 2) {
+        body(Main{}, g, fa0, fb0, fa1, fb1);
+        body(Main{}, g + 1, fa1, fb1, fa0, fb0);
+    }
+    for (; g < G; g += 2) {
+        body(Tail{}, g, fa0, fb0, fa1, fb1);
+        if (g + 1 < G) body(Tail{}, g + 1, fa1, fb1, fa0, fb0);
+    }
+    const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
+    if (key >= S) return;
+    const I8Out out = {Dpart + (size_t)(s * nsplit + slice) * R * S, nullptr, nullptr, nullptr};
+    i8_store<DA, NB>(acc, tr * (32 * WR) + wr * 32 + (lane >> 5) * 4, key, R, S, qmod(s), out);
+}
+
+// Sums the slices of k_contract_i8_split mod q per limb, then CRT and rounding.
+// All 3 nsplit loads of a thread are issued before the first is used (the
+// loop is unrolled to the 16-slice maximum), so the kernel waits for memory
+// once, not once per slice.
+constexpr int kMaxSlices = 16;
+__global__ void k_split_finish(const uint32_t *__restrict__ Dpart, int nsplit, size_t n, uint32_t *__restrict__ kval)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t v[3][kMaxSlices];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int j = 0; j < kMaxSlices; ++j) v[s][j] = j < nsplit ? Dpart[((size_t)s * nsplit + j) * n + t] : 0u;
+    uint32_t d[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        uint64_t sum = 0;  // at most 16 residues below 2^31
+#pragma unroll
+        for (int j = 0; j < kMaxSlices; ++j) sum += v[s][j];
+        d[s] = (uint32_t)(sum % qmod(s));
+    }
+    kval[t] = crt_decode_canon(d[0], d[1], d[2]);
+}
+
+// ---------------------------------------------------------------------------
 // Hybrid variant: one row block shared by WK waves, one key block per wave.
 // The row digits (4 fragments per k step, used by every wave) go through the
 // LDS ring by global_load_lds; each wave's key digits (used by that wave only)
@@ -569,20 +722,39 @@ __device__ __forceinline__ int digit_split(int &x)
     return d;
 }
 
-// Balanced int32 rows [3][cap][ldr] -> row digit store, rows [r0, r0 + R).
-// One thread per (limb, row, 4 consecutive k): one 32-bit word per digit plane.
-__global__ void k_rows_digits(const int32_t *__restrict__ a32, int cap, int ldr, int r0, int R, int ksteps,
-                              size_t limb_bytes, int8_t *__restrict__ dig)
+// Compact rows -> every row-store form in one pass (RowStore::put): rows
+// [r0, r0 + R) from c_L [R][3][N] and coefficient 0 of c_0..c_{src_l-1},
+// c0 [R][src_l][3] (words in [0, q]; q is taken as 0):
+//   A [3][cap][ldr]    balanced int32 (c_L, then the c0 columns, zero-padded),
+//   c0s [cap][l][3]    the c0 words as given (columns < src_l),
+//   dig                the int8 digit store (limb 0 in its three-digit window).
+// One thread per (limb, row, 4 consecutive columns of the ksteps x 32).
+__global__ void k_rows_put(const uint32_t *__restrict__ cl, const uint32_t *__restrict__ c0, int src_l, int R, int r0,
+                           int cap, int ldr, int l_store, int ksteps, size_t limb_bytes, int32_t *__restrict__ A,
+                           uint32_t *__restrict__ c0s, int8_t *__restrict__ dig)
 {
     const int quads = ksteps * 8;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)3 * R * quads) return;
     const int s = (int)(i / ((size_t)R * quads));
     const size_t rem = i % ((size_t)R * quads);
-    const int row = r0 + (int)(rem / quads), k = 4 * (int)(rem % quads);
+    const int rr = (int)(rem / quads), row = r0 + rr, k = 4 * (int)(rem % quads);
+    const uint32_t q = qmod(s);
     int v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = (k + u < ldr) ? a32[((size_t)s * cap + row) * ldr + k + u] : 0;
+    for (int u = 0; u < 4; ++u) {
+        const int col = k + u;
+        uint32_t t = 0;
+        if (col < IPFE_N) {
+            t = cl[((size_t)rr * 3 + s) * IPFE_N + col];
+        } else if (col - IPFE_N < src_l) {
+            t = c0[((size_t)rr * src_l + (col - IPFE_N)) * 3 + s];
+            c0s[((size_t)row * l_store + (col - IPFE_N)) * 3 + s] = t;
+        }
+        if (t >= q) t -= q;
+        v[u] = (t > q / 2) ? (int)((int64_t)t - q) : (int)t;
+        if (col < ldr) A[((size_t)s * cap + row) * ldr + col] = v[u];
+    }
     // limb 0 (q < 2^24): the representative in [-8421504, 8355711], the range
     // of three balanced base-256 digits, so digit 3 is zero
     if (s == 0) {
@@ -634,16 +806,18 @@ __global__ void k_keys_digits(const double *__restrict__ kd, int S, int K, int s
 size_t rows_digit_limb_bytes(int cap, int ksteps) { return (size_t)(cap + 128) / 32 * ksteps * DG_A * 1024; }
 size_t keys_digit_limb_bytes(int spad, int ksteps, int nb) { return (size_t)spad / 32 * ksteps * nb * 1024; }
 
-int launch_rows_digits(const int32_t *a32, int cap, int ldr, int r0, int R, int ksteps, int8_t *dig, hipStream_t st)
+int launch_rows_put(const uint32_t *cl, const uint32_t *c0, int src_l, int R, int r0, int cap, int ldr, int l_store,
+                    int ksteps, int32_t *A, uint32_t *c0s, int8_t *dig, hipStream_t st)
 {
     const size_t n = (size_t)3 * R * ksteps * 8;
     if (n == 0) return 0;
-    if (ksteps * 32 < ldr || r0 < 0 || r0 + R > cap) {
-        ipfe_set_error("row digits: %d k steps for %d columns, rows [%d, %d) of %d", ksteps, ldr, r0, r0 + R, cap);
+    if (ksteps * 32 < ldr || ldr < IPFE_N + src_l || src_l > l_store || r0 < 0 || r0 + R > cap) {
+        ipfe_set_error("row store: %d k steps for %d columns (%d slots of %d), rows [%d, %d) of %d", ksteps, ldr,
+                       src_l, l_store, r0, r0 + R, cap);
         return -1;
     }
-    hipLaunchKernelGGL(k_rows_digits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a32, cap, ldr, r0, R,
-                       ksteps, rows_digit_limb_bytes(cap, ksteps), dig);
+    hipLaunchKernelGGL(k_rows_put, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cl, c0, src_l, R, r0, cap,
+                       ldr, l_store, ksteps, rows_digit_limb_bytes(cap, ksteps), A, c0s, dig);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -750,6 +924,51 @@ static int persistent_grid(const void *fn, int block_threads, int ntiles)
     }
     const int tiles8 = (ntiles + 7) / 8 * 8;
     return std::max(8, std::min(nb * cus, tiles8) / 8 * 8);
+}
+
+// All three limbs of a small row range in one split-K launch plus the finish
+// kernel.  A: row digit store of limb 0 at row r0 (limb s at + s a_limb_bytes),
+// B: key digit store of limb 0 (limb s at + s b_limb_bytes), two key digits.
+// Dpart: 3 nsplit R S words of scratch, nsplit from split_k_slices.
+int split_k_slices(int R, int S, int ksteps)
+{
+    const int wr = R <= 32 ? 1 : 2, wk = R <= 32 ? 4 : 2;
+    const int tiles = ((R + 32 * wr - 1) / (32 * wr)) * ((S + 32 * wk - 1) / (32 * wk));
+    // about 2,048 blocks, at least 8 k steps per slice; every slice non-empty
+    const int n = std::max(1, std::min({2048 / std::max(1, 3 * tiles), kMaxSlices, ksteps / 8}));
+    const int kper = (ksteps + n - 1) / n;
+    return (ksteps + kper - 1) / kper;
+}
+
+int launch_contract_i8_split(const int8_t *A, size_t a_limb_bytes, int ksteps_a, const int8_t *B,
+                             size_t b_limb_bytes, int ksteps, int R, int S, uint32_t *Dpart, uint32_t *kval,
+                             hipStream_t st)
+{
+    if (R <= 0 || S <= 0) return 0;
+    if (ksteps < 1 || ksteps > ksteps_a || ksteps * 32 > (1 << 14) || R > 4096) {
+        ipfe_set_error("split int8 contraction: %d rows, %d k steps (row store %d)", R, ksteps, ksteps_a);
+        return -1;
+    }
+    const int nsplit = split_k_slices(R, S, ksteps), kper = (ksteps + nsplit - 1) / nsplit;
+    const bool narrow = R <= 32;
+    const int wr = narrow ? 1 : 2, wk = narrow ? 4 : 2;
+    const int ntk = (S + 32 * wk - 1) / (32 * wk), ntr = (R + 32 * wr - 1) / (32 * wr), ntiles = ntk * ntr;
+    const dim3 grid((unsigned)(3 * nsplit * ntiles)), block(64 * wr * wk);
+    const long long as = (long long)ksteps_a * DG_A * FRAG_V4, bs = (long long)ksteps * 2 * FRAG_V4;
+    const v4i *a = reinterpret_cast<const v4i *>(A);
+    const v4i *b = reinterpret_cast<const v4i *>(B);
+    const long long al = (long long)(a_limb_bytes / 16), bl = (long long)(b_limb_bytes / 16);
+    if (narrow)
+        hipLaunchKernelGGL((k_contract_i8_split<1, 4, 3>), grid, block, 0, st, a, al, b, bl, R, S, ksteps, as, bs,
+                           nsplit, kper, ntk, ntiles, Dpart);
+    else
+        hipLaunchKernelGGL((k_contract_i8_split<2, 2, 3>), grid, block, 0, st, a, al, b, bl, R, S, ksteps, as, bs,
+                           nsplit, kper, ntk, ntiles, Dpart);
+    HIP_OK(hipGetLastError());
+    const size_t n = (size_t)R * S;
+    hipLaunchKernelGGL(k_split_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Dpart, nsplit, n, kval);
+    HIP_OK(hipGetLastError());
+    return 0;
 }
 
 // A: row digit store of one limb at row r0 (r0 a multiple of 32; the store

@@ -61,44 +61,6 @@ __global__ void k_contract_keys(const uint32_t *__restrict__ sk, int S, int K, i
     if ((threadIdx.x & 63) == 0) atomicMax(maxabs + s, a);
 }
 
-// rows c_L [R][3][N] canonical (row_stride words between rows) -> int32 balanced,
-// written to out[(s * cap + r0 + r) * ldr + j] (the row store is [3][cap][ldr]).
-__global__ void k_rows_balanced(const uint32_t *__restrict__ cl, size_t row_stride, int R, int32_t *__restrict__ out,
-                                int cap, int r0, int ldr)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t per_row = (size_t)3 * IPFE_N;
-    if (i >= (size_t)R * per_row) return;
-    const int r = (int)(i / per_row);
-    const int rem = (int)(i % per_row);
-    const int s = rem / IPFE_N, j = rem % IPFE_N;
-    const uint32_t q = qmod(s);
-    uint32_t t = cl[(size_t)r * row_stride + rem];
-    if (t >= q) t -= q;
-    int64_t v = (t > q / 2) ? (int64_t)t - q : (int64_t)t;
-    out[((size_t)s * cap + r0 + r) * ldr + j] = (int32_t)v;
-}
-
-// Columns N .. ldr-1 of the row store: coefficient 0 of c_0 .. c_{l-1}
-// (c0 [R][l][3] canonical), balanced, zero-padded.
-__global__ void k_rows_c0(const uint32_t *__restrict__ c0, int R, int l, int32_t *__restrict__ out, int cap, int r0,
-                          int ldr)
-{
-    const int w = ldr - IPFE_N;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)R * 3 * w) return;
-    const int r = (int)(i / (3 * (size_t)w));
-    const int rem = (int)(i % (3 * (size_t)w));
-    const int s = rem / w, col = rem % w;
-    const uint32_t q = qmod(s);
-    int64_t v = 0;
-    if (col < l) {
-        uint32_t t = c0[((size_t)r * l + col) * 3 + s];
-        if (t >= q) t -= q;
-        v = (t > q / 2) ? (int64_t)t - q : (int64_t)t;
-    }
-    out[((size_t)s * cap + r0 + r) * ldr + IPFE_N + col] = (int32_t)v;
-}
 
 // ---------------------------------------------------------------------------
 // Contraction: D[r][v] = sum_j A[r][j] B[v][j] mod q for one limb.
@@ -439,6 +401,9 @@ __global__ __launch_bounds__(256) void k_decide_wave(const uint32_t *__restrict_
             double sum = 0.0;
             if (EXACT_INT) {
                 double part = 0.0;
+                // several loads in flight per lane: small batches wait on
+                // memory latency, not bandwidth
+#pragma unroll 8
                 for (int t = lane; t < total; t += 64) part += term(t);
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
@@ -521,25 +486,6 @@ int launch_contract_keys(const uint32_t *sk, int S, int K, int sign, const Spars
     return 0;
 }
 
-int launch_rows_c0(const uint32_t *c0, int R, int l, int32_t *out, int cap, int r0, int ldr, hipStream_t st)
-{
-    const size_t n = (size_t)R * 3 * (ldr - IPFE_N);
-    if (n == 0) return 0;
-    hipLaunchKernelGGL(k_rows_c0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c0, R, l, out, cap, r0, ldr);
-    HIP_OK(hipGetLastError());
-    return 0;
-}
-
-int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, int cap, int r0, int ldr,
-                         hipStream_t st)
-{
-    if (R <= 0) return 0;
-    const size_t n = (size_t)R * 3 * IPFE_N;
-    hipLaunchKernelGGL(k_rows_balanced, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cl, row_stride, R, out,
-                       cap, r0, ldr);
-    HIP_OK(hipGetLastError());
-    return 0;
-}
 
 int launch_contract(const int32_t *A, const double *B, int R, int S, int K, int lda, int ldb, int s, int kb, uint32_t *D,
                     hipStream_t st)

@@ -19,9 +19,6 @@ struct SvmModelDev {
 
 int launch_contract_keys(const uint32_t *sk, int S, int K, int sign, const SparseY &y, double *kd, unsigned int *maxabs,
                          hipStream_t st);
-int launch_rows_balanced(const uint32_t *cl, size_t row_stride, int R, int32_t *out, int cap, int r0, int ldr,
-                         hipStream_t st);
-int launch_rows_c0(const uint32_t *c0, int R, int l, int32_t *out, int cap, int r0, int ldr, hipStream_t st);
 int launch_contract_slow(const int32_t *A, const double *B, int R, int S, int K, int lda, int ldb, int s,
                          const BarrettConsts &bc, uint32_t *D, hipStream_t st);
 int launch_contract(const int32_t *A, const double *B, int R, int S, int K, int lda, int ldb, int s, int kb, uint32_t *D,
@@ -34,11 +31,18 @@ int launch_contract_sgpr(const double *A, const double *B, int R, int S, int K, 
 // MFMA contraction of one limb.
 size_t rows_digit_limb_bytes(int cap, int ksteps);
 size_t keys_digit_limb_bytes(int spad, int ksteps, int nb);
-int launch_rows_digits(const int32_t *a32, int cap, int ldr, int r0, int R, int ksteps, int8_t *dig, hipStream_t st);
+int launch_rows_put(const uint32_t *cl, const uint32_t *c0, int src_l, int R, int r0, int cap, int ldr, int l_store,
+                    int ksteps, int32_t *A, uint32_t *c0s, int8_t *dig, hipStream_t st);
 int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int nb, int8_t *dig, hipStream_t st);
 int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
                        uint32_t *D, hipStream_t st, int variant = 0, const uint32_t *crt_d0 = nullptr,
                        const uint32_t *crt_d1 = nullptr, uint32_t *kval = nullptr);
+// Small row ranges: the three limbs and slices of the k steps as blocks of one
+// launch (two key digits), then the slices' sum, CRT and rounding into kval.
+int split_k_slices(int R, int S, int ksteps);
+int launch_contract_i8_split(const int8_t *A, size_t a_limb_bytes, int ksteps_a, const int8_t *B,
+                             size_t b_limb_bytes, int ksteps, int R, int S, uint32_t *Dpart, uint32_t *kval,
+                             hipStream_t st);
 int i8_shape_count();
 void i8_shape(int variant, int *wr, int *wk);
 int i8_variant_stages(int variant);

@@ -299,11 +299,8 @@ int RowStore::put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev,
         return -1;
     }
     if (n == 0) return 0;
-    if (launch_rows_balanced(cl_dev, kPoly, n, A.get(), cap, r0, ldr, st)) return -1;
-    if (launch_rows_c0(c0_dev, n, src_l, A.get(), cap, r0, ldr, st)) return -1;
-    HIP_OK(hipMemcpy2DAsync(c0.get() + (size_t)r0 * l * 3, (size_t)l * 12, c0_dev, (size_t)src_l * 12,
-                            (size_t)src_l * 12, n, hipMemcpyDeviceToDevice, st));
-    if (launch_rows_digits(A.get(), cap, ldr, r0, n, ksteps, Adig.get(), st)) return -1;
+    if (launch_rows_put(cl_dev, c0_dev, src_l, n, r0, cap, ldr, l, ksteps, A.get(), c0.get(), Adig.get(), st))
+        return -1;
     for (int b = r0 / 64; b <= (r0 + n - 1) / 64; ++b) a64_ok[b] = 0;
     return 0;
 }
@@ -328,11 +325,21 @@ int RowStore::ensure_a64(int r0, int R, hipStream_t st) const
     return 0;
 }
 
+// Row ranges of at most this many (row, key) pairs run the split-K
+// contraction (serving batches); above it the per-limb kernels are as fast.
+constexpr size_t kSplitMaxPairs = (size_t)1 << 19;
+
+size_t Scratch::split_bytes(int rows, int S) { return (size_t)48 * std::min((size_t)rows * S, kSplitMaxPairs) * 4; }
+
 int Scratch::reserve(int rows, int S)
 {
     if (D.n < (size_t)3 * rows * S || kval.n < (size_t)rows * S || labels.n < (size_t)rows) {
         if (D.alloc((size_t)3 * rows * S) || kval.alloc((size_t)rows * S) || labels.alloc(rows)) return -1;
     }
+    // split-K slices of the small row ranges that fit: up to 3 x 16 x the
+    // pairs (allocated here, not while serving)
+    const size_t part = split_bytes(rows, S) / 4;
+    if (Dpart.n < part && Dpart.alloc(part)) return -1;
     if (!flag.get() && flag.alloc(1)) return -1;
     return 0;
 }
@@ -340,6 +347,7 @@ int Scratch::reserve(int rows, int S)
 // ---------------------------------------------------------------------------
 // Prediction
 // ---------------------------------------------------------------------------
+
 int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratch &sc, int scratch_rows, int mode,
                  double *labels_host, uint32_t *kval_host, PredictTimers *tm, hipStream_t st)
 {
@@ -372,7 +380,8 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
     const int chunk = std::max(64, scratch_rows / 64 * 64);
     if (sc.reserve(std::min(chunk, ((nrows + 63) / 64) * 64), S)) return -1;
     if (tm) tm->ms[0] = tm->ms[1] = tm->ms[2] = tm->limb_ms[0] = tm->limb_ms[1] = tm->limb_ms[2] = 0;
-    if (launch_fill_i32(sc.flag.get(), 0, 1, st)) return -1;
+    // the flag is raised only by the separate key-value fold (unfused keys)
+    if (!m.fused && launch_fill_i32(sc.flag.get(), 0, 1, st)) return -1;
     for (int r0 = row0; r0 < row0 + nrows; r0 += chunk) {
         const int R = std::min(chunk, row0 + nrows - r0);
         if (tm) HIP_OK(hipEventRecord(tm->ev[0], st));
@@ -380,7 +389,24 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
         // CRT and rounding of the fused keys' key values
         const bool i8 = mode == 2 && m.i8 && r0 % 32 == 0;
         const bool fuse_crt = i8 && m.fused && !getenv("CUFE_NO_FUSED_CRT");
-        for (int s = 0; s < 3; ++s) {
+        // a small row range: all limbs and slices of the k steps in one launch
+        const bool split = fuse_crt && m.nb == 2 && (size_t)R * S <= kSplitMaxPairs && !getenv("CUFE_NO_SPLIT_K");
+        if (split) {
+            // at most 16 slices: Scratch::reserve holds 48 x 2^19 words
+            const size_t need = (size_t)3 * split_k_slices(R, S, m.ksteps) * R * S;
+            if (sc.Dpart.n < need && sc.Dpart.alloc(need)) return -1;
+            if (launch_contract_i8_split(rs.Adig.get() + (size_t)(r0 / 32) * rs.ksteps * 4 * 1024,
+                                         rows_digit_limb_bytes(rs.cap, rs.ksteps), rs.ksteps, m.kdig.get(),
+                                         keys_digit_limb_bytes(m.spad, m.ksteps, m.nb), m.ksteps, R, S,
+                                         sc.Dpart.get(), sc.kval.get(), st))
+                return -1;
+            // the limbs run together: the whole time counts as limb 0's
+            if (tm) {
+                HIP_OK(hipEventRecord(tm->ev[4], st));
+                HIP_OK(hipEventRecord(tm->ev[5], st));
+            }
+        }
+        for (int s = 0; s < 3 && !split; ++s) {
             const int32_t *As = rs.A.get() + ((size_t)s * rs.cap + r0) * rs.ldr;
             const double *Bs = m.kd.get() + (size_t)s * S * m.kkey;
             uint32_t *Ds = sc.D.get() + (size_t)s * R * S;
@@ -406,7 +432,8 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
         }
         if (tm) HIP_OK(hipEventRecord(tm->ev[1], st));
         if (fuse_crt) {
-            // key values already written by limb 2's contraction
+            // key values already written by limb 2's contraction (or the split
+            // launch's finish kernel)
         } else if (m.fused) {
             if (launch_crt_kvalue(sc.D.get(), (size_t)R * S, sc.kval.get(), st)) return -1;
         } else if (launch_kvalue(sc.D.get(), rs.c0.get() + (size_t)r0 * rs.l * 3, rs.l, m.y, R, S, sc.kval.get(),
@@ -698,7 +725,7 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
     if (predict_rows(c->model, c->rows, row0, nrows, c->scratch, chunk_rows, contract_mode(), labels, kvalues, &tm, st))
         return -1;
     int flag = 0;
-    HIP_OK(hipMemcpyAsync(&flag, c->scratch.flag.get(), 4, hipMemcpyDeviceToHost, st));
+    if (!c->model.fused) HIP_OK(hipMemcpyAsync(&flag, c->scratch.flag.get(), 4, hipMemcpyDeviceToHost, st));
     HIP_OK(hipEventRecord(c->ev[5], st));
     HIP_OK(hipEventSynchronize(c->ev[5]));
     float tot = 0;

@@ -66,9 +66,12 @@ struct RowStore {
 
 struct Scratch {
     DevBuf<uint32_t> D, kval;
+    DevBuf<uint32_t> Dpart;  // split-K slices of small row ranges
     DevBuf<double> labels;
     DevBuf<int> flag;
     int reserve(int rows, int S);
+    // device bytes of Dpart that reserve(rows, S) allocates
+    static size_t split_bytes(int rows, int S);
 };
 
 // Predicts rows [r0, r0+R) of the store against the model: labels and

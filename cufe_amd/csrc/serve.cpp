@@ -85,6 +85,9 @@ struct cufe_server {
     size_t budget = 0;
     int max_batch = 0;
     int batching = 1;
+    // worker's poll interval while a batch is in flight (us; 0 = spin);
+    // CUFE_SERVE_POLL_US overrides
+    int poll_us = 50;
     std::vector<std::unique_ptr<Model>> models;
     size_t model_bytes = 0;
     int S_max = 0, l_max = 0, cap = 0;
@@ -180,7 +183,11 @@ int launch_batch(cufe_server *s, Slot &sl)
         if (predict_rows(*gr.m, sl.rows, gr.r0, gr.R, sl.sc, s->cap, mode, sl.h_labels + gr.r0,
                          gr.want_kv ? sl.h_kval + gr.kv_off : nullptr, nullptr, sl.st))
             return -1;
-        HIP_OK(hipMemcpyAsync(sl.h_flags + g, sl.sc.flag.get(), 4, hipMemcpyDeviceToHost, sl.st));
+        // the flag is raised only by the separate key-value fold (unfused keys)
+        if (gr.m->fused)
+            sl.h_flags[g] = 0;
+        else
+            HIP_OK(hipMemcpyAsync(sl.h_flags + g, sl.sc.flag.get(), 4, hipMemcpyDeviceToHost, sl.st));
     }
     return 0;
 }
@@ -302,7 +309,13 @@ void worker_main(cufe_server *s)
         std::unique_lock<std::mutex> lk(s->mu);
         if (s->pending.empty()) {
             if (s->stop && !any_busy) return;
-            s->cv_submit.wait_for(lk, std::chrono::microseconds(any_busy ? 50 : 2000));
+            if (any_busy && s->poll_us == 0) {
+                // a batch in flight: poll its event without sleeping
+                lk.unlock();
+                std::this_thread::yield();
+            } else {
+                s->cv_submit.wait_for(lk, std::chrono::microseconds(any_busy ? s->poll_us : 2000));
+            }
         }
     }
 }
@@ -385,6 +398,7 @@ extern "C" int cufe_server_create(cufe_ctx *ctx, size_t hbm_budget, int max_batc
     s->budget = hbm_budget ? hbm_budget : ((size_t)16 << 30);
     s->max_batch = max_batch_rows > 0 ? max_batch_rows : 4096;
     s->batching = batching ? 1 : 0;
+    if (const char *e = getenv("CUFE_SERVE_POLL_US")) s->poll_us = std::max(0, atoi(e));
     s->l_max = ctx_key_len(ctx);
     *out = s;
     return 0;
@@ -416,7 +430,8 @@ extern "C" int cufe_server_start(cufe_server *s)
     }
     const size_t per_row = RowStore::bytes_per_row(s->l_max) + (kPoly + (size_t)s->l_max * 3) * 4 +
                            (size_t)s->S_max * 16 + 8;
-    const size_t reserve = (size_t)64 << 20;
+    // 64 MiB of headroom plus both slots' split-K scratch (fixed size)
+    const size_t reserve = ((size_t)64 << 20) + 2 * Scratch::split_bytes(s->max_batch, s->S_max);
     if (s->budget < s->model_bytes + reserve + 2 * 64 * per_row) {
         ipfe_set_error("cufe_server_start: HBM budget %zu below models (%zu) + two 64-row slots (%zu)", s->budget,
                        s->model_bytes, 2 * 64 * per_row);

@@ -82,6 +82,38 @@ def test_kernels_agree(gpu, hi, digits):
     ctx.close()
 
 
+def test_split_k_small_batches_agree(gpu):
+    """Row ranges of at most 1,024 rows run all limbs and slices of the k steps
+    as one split-K launch plus a finish kernel (sum of slices mod q, CRT);
+    every size, including ragged and unaligned ones, gives the bits of the
+    per-limb kernels."""
+    rng = np.random.default_rng(33)
+    l, R, S = 124, 1100, 300
+    ctx = gpu.Context()
+    ctx.setup(h(b"kA"), h(b"kS"), h(b"kE"), 785)
+    x = rng.integers(0, 5, (R, l)).astype(np.uint32)
+    ctx.rows_reserve(R, l)
+    ctx.encrypt_rows(x, h(b"k2"), h(b"k3"), nonce0=0)
+    ctx.set_model(model_with(gpu, rng, S, l, 2))
+    assert ctx.contract_digits() == 2
+    _, kv_all = predict_with(ctx, R, {})  # 1,100 rows: one chunk above the split size
+    for r0, n in ((0, 1), (0, 32), (0, 33), (64, 200), (37, 5), (0, 1024), (70, 1024)):
+        lab_s, kv_s, _ = ctx.predict(r0, n, want_kvalues=True)
+        old = os.environ.get("CUFE_NO_SPLIT_K")
+        os.environ["CUFE_NO_SPLIT_K"] = "1"
+        try:
+            lab_p, kv_p, _ = ctx.predict(r0, n, want_kvalues=True)
+        finally:
+            if old is None:
+                os.environ.pop("CUFE_NO_SPLIT_K", None)
+            else:
+                os.environ["CUFE_NO_SPLIT_K"] = old
+        note(f"split-K rows [{r0}, {r0 + n})")
+        assert np.array_equal(kv_s, kv_p) and np.array_equal(lab_s, lab_p), (r0, n)
+        assert np.array_equal(kv_s, kv_all[r0:r0 + n]), (r0, n)
+    ctx.close()
+
+
 def test_large_keys_fall_back_to_fp64(gpu, po):
     rng = np.random.default_rng(32)
     l, R, S = 40, 5, 6

@@ -325,11 +325,18 @@ int RowStore::ensure_a64(int r0, int R, hipStream_t st) const
     return 0;
 }
 
-// Row ranges of at most this many (row, key) pairs run the split-K
-// contraction (serving batches); above it the per-limb kernels are as fast.
+// Row ranges of at most kSplitMaxRows rows and kSplitMaxPairs (row, key)
+// pairs run the split-K contraction (serving batches): there the per-limb
+// kernels leave most CUs idle.  Larger ranges fill the GPU by themselves, and
+// under load the split launch slows the other slot's contraction more than it
+// gains.
+constexpr int kSplitMaxRows = 256;
 constexpr size_t kSplitMaxPairs = (size_t)1 << 19;
 
-size_t Scratch::split_bytes(int rows, int S) { return (size_t)48 * std::min((size_t)rows * S, kSplitMaxPairs) * 4; }
+size_t Scratch::split_bytes(int rows, int S)
+{
+    return (size_t)48 * std::min((size_t)std::min(rows, kSplitMaxRows) * S, kSplitMaxPairs) * 4;
+}
 
 int Scratch::reserve(int rows, int S)
 {
@@ -390,7 +397,8 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
         const bool i8 = mode == 2 && m.i8 && r0 % 32 == 0;
         const bool fuse_crt = i8 && m.fused && !getenv("CUFE_NO_FUSED_CRT");
         // a small row range: all limbs and slices of the k steps in one launch
-        const bool split = fuse_crt && m.nb == 2 && (size_t)R * S <= kSplitMaxPairs && !getenv("CUFE_NO_SPLIT_K");
+        const bool split = fuse_crt && sc.split_ok && m.nb == 2 && R <= kSplitMaxRows &&
+                           (size_t)R * S <= kSplitMaxPairs && !getenv("CUFE_NO_SPLIT_K");
         if (split) {
             // at most 16 slices: Scratch::reserve holds 48 x 2^19 words
             const size_t need = (size_t)3 * split_k_slices(R, S, m.ksteps) * R * S;

@@ -69,6 +69,9 @@ struct Scratch {
     DevBuf<uint32_t> Dpart;  // split-K slices of small row ranges
     DevBuf<double> labels;
     DevBuf<int> flag;
+    // small row ranges may use the split-K launch (a server turns it off while
+    // its other slot runs: there it would slow that slot's contraction)
+    bool split_ok = true;
     int reserve(int rows, int S);
     // device bytes of Dpart that reserve(rows, S) allocates
     static size_t split_bytes(int rows, int S);

@@ -74,6 +74,7 @@ struct Slot {
     std::vector<int> req_row, req_group;
     std::vector<Group> groups;
     bool busy = false;
+    int batch_rows = 0;  // rows of the batch in flight
     bool failed = false;
     std::string err;
 };
@@ -178,6 +179,12 @@ int launch_batch(cufe_server *s, Slot &sl)
                         r.l))
             return -1;
     }
+    // small groups take the low-latency split-K launch unless the other slot
+    // runs a large batch (under load the split launch would slow it more than
+    // it gains)
+    const Slot &other = (&sl == &s->slots[0]) ? s->slots[1] : s->slots[0];
+    sl.batch_rows = row;
+    sl.sc.split_ok = !other.busy || other.batch_rows <= 256;
     for (size_t g = 0; g < sl.groups.size(); ++g) {
         const Group &gr = sl.groups[g];
         if (predict_rows(*gr.m, sl.rows, gr.r0, gr.R, sl.sc, s->cap, mode, sl.h_labels + gr.r0,
@@ -455,13 +462,19 @@ extern "C" int cufe_server_start(cufe_server *s)
     }
     HIP_OK(hipDeviceSynchronize());
     // Run every model once per power-of-two batch size on the (zeroed) rows of
-    // slot 0, so the contraction's block shapes are tuned here and not on the
-    // first requests.
+    // slot 0, with and without the split-K launch for small batches (a batch
+    // may take either), so the contraction's block shapes are tuned here and
+    // not on the first requests.
     Slot &w = s->slots[0];
     const int mode = contract_mode();
-    for (auto &m : s->models)
-        for (int R = 1; R <= s->cap; R *= 2)
-            if (predict_rows(*m, w.rows, 0, R, w.sc, s->cap, mode, w.h_labels, nullptr, nullptr, w.st)) return -1;
+    for (int split_ok = 1; split_ok >= 0; --split_ok) {
+        w.sc.split_ok = split_ok != 0;
+        for (auto &m : s->models)
+            for (int R = 1; R <= s->cap; R *= 2)
+                if (predict_rows(*m, w.rows, 0, R, w.sc, s->cap, mode, w.h_labels, nullptr, nullptr, w.st))
+                    return -1;
+    }
+    w.sc.split_ok = true;
     HIP_OK(hipStreamSynchronize(w.st));
     s->started = true;
     s->worker = std::thread(worker_main, s);

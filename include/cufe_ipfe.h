@@ -187,8 +187,9 @@ int cufe_ctx_rows_count(const cufe_ctx *ctx);
 int cufe_ctx_predict(cufe_ctx *ctx, int row0, int nrows, double *labels, uint32_t *kvalues, float *ms);
 /* Contraction time per limb (ms[3], summed over row chunks) of the last
  * cufe_ctx_predict call (limb 0 may run a cheaper kernel: its residues fit
- * three int8 digits).  Chunks of at most 1,024 rows run the three limbs in
- * one split-K launch, whose whole time counts as limb 0's. */
+ * three int8 digits).  Chunks of at most 256 rows (and 2^19 row-key pairs)
+ * run the three limbs in one split-K launch, whose whole time counts as
+ * limb 0's. */
 int cufe_ctx_contract_limb_ms(const cufe_ctx *ctx, float *ms);
 /* Reduction interval of the fp64 contraction and the largest balanced
  * functional-key coefficient, per limb. */

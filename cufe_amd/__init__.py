@@ -65,7 +65,7 @@ def lib() -> C.CDLL:
         "cufe_microbench": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
         "cufe_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
         "cufe_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
-        "cufe_contract_variants": (C.c_int, [C.c_int, C.POINTER(C.c_uint)]),
+        "cufe_contract_variants": (C.c_int, [C.c_int, C.POINTER(C.c_ulonglong)]),
         "cufe_wire_size": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
         "cufe_wire_pack": (C.c_int, [C.c_int, _u32p, C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_void_p,
                                      C.c_size_t]),
@@ -313,7 +313,7 @@ def wire_parse(blob) -> dict:
 
 def contract_variants(nb: int) -> list:
     """Indices of the int8 contraction variants that run with nb key digits."""
-    m = C.c_uint()
+    m = C.c_ulonglong()
     n = lib().cufe_contract_variants(nb, C.byref(m))
     return [v for v in range(n) if (m.value >> v) & 1]
 

@@ -89,6 +89,47 @@ __device__ __forceinline__ void i8_store(const v16i (&acc)[DA][NB], int row0, in
     }
 }
 
+// Epilogue of the merged form: acc[s][i] = C_s = sum over d + e = s of P_de
+// (products of equal weight 256^s share one accumulator; |C_s| <= min(DA, NB)
+// K 2^14 <= 3 2^28 < 2^31 for K <= 2^14).  sum_s 256^s C_s is formed from the
+// top by Horner steps in doubles, reduced mod q after every two steps: a
+// reduced value is below 2^31, so two steps stay below 2^48 and the first
+// (from |C| < 2^30) below 2^46 -- every value exact.
+template <int DA, int NB>
+__device__ __forceinline__ void i8_store_merged(const v16i (&acc)[DA + NB - 1], int row0, int key, int R, int S,
+                                                uint32_t q, const I8Out &out)
+{
+    constexpr int NS = DA + NB - 1;
+    const double qd = (double)q, qinv = 1.0 / qd;
+    auto reduce = [&](double t) {
+        double m = fma(-floor(t * qinv), qd, t);
+        m += (m < 0.0) ? qd : 0.0;
+        m -= (m >= qd) ? qd : 0.0;
+        return m;
+    };
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = row0 + (i >> 2) * 8 + (i & 3);
+        if (row >= R) continue;
+        double T = (double)acc[NS - 1][i];
+        int steps = 0;
+#pragma unroll
+        for (int s = NS - 2; s >= 0; --s) {
+            T = fma(T, 256.0, (double)acc[s][i]);
+            if (++steps == 2 || s == 0) {
+                T = reduce(T);
+                steps = 0;
+            }
+        }
+        const uint32_t res = (uint32_t)T;
+        const size_t idx = (size_t)row * S + key;
+        if (out.kval)
+            out.kval[idx] = crt_decode_canon(out.d0[idx], out.d1[idx], res);
+        else
+            out.D[idx] = res;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Block tile: WR row blocks x WK key blocks of 32, one wave per 32 x 32 output
 // block with DG_A x DG_B int32 accumulators.  Per k step the block stages the
@@ -305,6 +346,13 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 // queued MFMAs instead of between steps, and a second register set holds the
 // next step's fragments.
 //
+// MG: the products of equal weight 256^(d + e) add into one accumulator
+// (DA + NB - 1 of them instead of DA NB), which frees the registers for
+// KW = 2: each wave then computes a 32 x 64 output block from 4 key
+// fragments and DA row fragments per step -- a third fewer LDS reads per
+// MFMA than 32 x 32 -- and, at two 4-wave blocks per CU, one block's
+// epilogue runs while the other's MFMAs keep the matrix pipe busy.
+//
 // DA = 3 uses only row digits 0..2: valid when every row value lies in
 // [-8421504, 8355711] (digit 3 is then zero), which the row digit store
 // guarantees for limb 0 (q < 2^24).  The zero digit plane is still loaded, so
@@ -313,20 +361,21 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 // The main loop runs while the ring still has steps to load; the last
 // STAGES + 1 steps (at most) wait with the conservative vmcnt(0).
 // ---------------------------------------------------------------------------
-template <int WR, int WK, int NB, int STAGES, int DA, bool PT>
-__global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *__restrict__ A,
+template <int WR, int WK, int NB, int STAGES, int DA, bool PT, bool MG, int OCC, int KW>
+__global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i *__restrict__ A,
                                                                     const v4i *__restrict__ B, int R, int S,
                                                                     int ksteps, long long a_stride,
                                                                     long long b_stride, uint32_t q,
                                                                     I8Out out, int ntk, int ntiles)
 {
     constexpr int W = WR * WK;
-    constexpr int F = WR * DG_A + WK * NB;
+    constexpr int F = WR * DG_A + WK * KW * NB;
     static_assert(F % W == 0, "fragments must split evenly over the waves");
     static_assert(DA * NB >= 5 && DA <= DG_A, "the interleave needs five MFMAs per step");
+    static_assert(KW == 1 || MG, "a wave of KW > 1 key blocks needs the merged accumulators' registers");
     constexpr int FW = F / W;
     constexpr int STEP_V4 = F * FRAG_V4;
-    constexpr int NM = DA * NB;  // MFMAs per step
+    constexpr int NM = DA * NB * KW;  // MFMAs per step
     __shared__ v4i ring[STAGES * STEP_V4];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -350,6 +399,8 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
     if (nt == 0) return;
     // the block's k steps in order: step g = tile g / ksteps, k step g mod ksteps
     const int G = nt * ksteps;
+    // fragment bases and strides are wave-uniform (scalar registers); each
+    // load adds the lane
     long long step[FW];
     const v4i *base[FW];
     auto set_base = [&](int j) {
@@ -358,11 +409,11 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
         for (int u = 0; u < FW; ++u) {
             const int f = w + u * W;
             if (f < WR * DG_A) {
-                base[u] = A + (size_t)(tr * WR + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4 + lane;
+                base[u] = A + (size_t)(tr * WR + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4;
                 step[u] = DG_A * FRAG_V4;
             } else {
                 const int fb = f - WR * DG_A;
-                base[u] = B + (size_t)(tk * WK + fb / NB) * b_stride + (fb % NB) * FRAG_V4 + lane;
+                base[u] = B + (size_t)(tk * WK * KW + fb / NB) * b_stride + (fb % NB) * FRAG_V4;
                 step[u] = NB * FRAG_V4;
             }
         }
@@ -374,7 +425,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
 #pragma unroll
         for (int u = 0; u < FW; ++u) {
             const int f = w + u * W;
-            __builtin_amdgcn_global_load_lds((glb_void_t *)(base[u] + (size_t)it_ks * step[u]),
+            __builtin_amdgcn_global_load_lds((glb_void_t *)(base[u] + (size_t)it_ks * step[u] + lane),
                                              (lds_void_t *)(slot + f * FRAG_V4), 16, 0, 0);
         }
         it_slot = it_slot + 1 == STAGES ? 0 : it_slot + 1;
@@ -384,12 +435,13 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
         }
     };
     const int a_off = wr * DG_A * FRAG_V4 + lane;
-    const int b_off = (WR * DG_A + wk * NB) * FRAG_V4 + lane;
-    v16i acc[DA][NB];
+    const int b_off = (WR * DG_A + wk * KW * NB) * FRAG_V4 + lane;
+    // per key block of the wave, MG: one accumulator per weight 256^(d + e);
+    // else one per (d, e)
+    constexpr int NACC = MG ? DA + NB - 1 : DA * NB;
+    v16i acc[KW * NACC];
 #pragma unroll
-    for (int d = 0; d < DA; ++d)
-#pragma unroll
-        for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+    for (int c = 0; c < KW * NACC; ++c) acc[c] = v16i{};
 #pragma unroll
     for (int p = 0; p < STAGES; ++p)
         if (p < G) issue_next();
@@ -398,22 +450,27 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
     else
         wait_vmcnt<0>();
     asm volatile("s_barrier" ::: "memory");
-    v4i fa0[DA], fb0[NB], fa1[DA], fb1[NB];
+    v4i fa0[DA], fb0[KW * NB], fa1[DA], fb1[KW * NB];
 #pragma unroll
     for (int d = 0; d < DA; ++d) fa0[d] = ring[a_off + d * FRAG_V4];
 #pragma unroll
-    for (int e = 0; e < NB; ++e) fb0[e] = ring[b_off + e * FRAG_V4];
+    for (int e = 0; e < KW * NB; ++e) fb0[e] = ring[b_off + e * FRAG_V4];
     int rd_slot = 1 % STAGES;  // slot of step g + 1
     // One step g.  In the last step the reads of "step G" fetch a slot no one
     // loads any more: in bounds, and never used.  Every wave runs the same
     // number of barriers.
-    auto body = [&](auto main, int g, v4i (&fa)[DA], v4i (&fb)[NB], v4i (&na)[DA], v4i (&nb)[NB]) {
+    auto body = [&](auto main, int g, v4i (&fa)[DA], v4i (&fb)[KW * NB], v4i (&na)[DA], v4i (&nb)[KW * NB]) {
         constexpr bool M = decltype(main)::value;
         const v4i *nx = ring + rd_slot * STEP_V4;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            acc[i / NB][i % NB] =
-                __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i / NB], fb[i % NB], acc[i / NB][i % NB], 0, 0, 0);
+            // MFMA i: key block kb (alternating), row digit d, key digit e
+            // (reversed on odd d): two consecutive MFMAs never add into the
+            // same accumulator
+            const int kb = i % KW, j = i / KW;
+            const int d = j / NB, e = (MG && (d & 1)) ? NB - 1 - j % NB : j % NB;
+            const int c = kb * NACC + (MG ? d + e : j);
+            acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d], fb[kb * NB + e], acc[c], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
             if (i == 1) {
                 if constexpr (M)
@@ -428,7 +485,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
                     issue_next();
             } else if (i == 3) {
 #pragma unroll
-                for (int e = 0; e < NB; ++e) nb[e] = nx[b_off + e * FRAG_V4];
+                for (int e = 0; e < KW * NB; ++e) nb[e] = nx[b_off + e * FRAG_V4];
             } else if (i >= 4) {
                 // row fragment i - 4; the last MFMA also takes any left over
 #pragma unroll
@@ -441,8 +498,17 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
     };
     auto store = [&](int j) {
         const int t = t0 + j * tstride, tk = t % ntk, tr = t / ntk;
-        const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
-        if (key < S) i8_store<DA, NB>(acc, tr * (32 * WR) + wr * 32 + (lane >> 5) * 4, key, R, S, q, out);
+        const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
+#pragma unroll
+        for (int kb = 0; kb < KW; ++kb) {
+            const int key = tk * (32 * WK * KW) + (wk * KW + kb) * 32 + (lane & 31);
+            if (key >= S) continue;
+            if constexpr (MG)
+                i8_store_merged<DA, NB>(reinterpret_cast<const v16i(&)[NACC]>(acc[kb * NACC]), row0, key, R, S, q,
+                                        out);
+            else
+                i8_store<DA, NB>(reinterpret_cast<const v16i(&)[DA][NB]>(acc[kb * NACC]), row0, key, R, S, q, out);
+        }
     };
     // after step g of a persistent block: at the end of a tile, store it and
     // restart the sums (a one-tile block stores after its loop)
@@ -452,9 +518,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_il(const v4i *_
             if (++c_ks < ksteps) return;
             store(c_j);
 #pragma unroll
-            for (int d = 0; d < DA; ++d)
-#pragma unroll
-                for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+            for (int c = 0; c < KW * NACC; ++c) acc[c] = v16i{};
             c_ks = 0;
             ++c_j;
         }
@@ -838,10 +902,15 @@ int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int
 
 // Kernel variants the launcher can run: row blocks x key blocks per block
 // tile, the LDS ring depth (0: the register-staged double-buffered kernel),
-// the kernel, and the row digits the MFMAs use (3: limb 0 only).
+// the kernel, the row digits the MFMAs use (3: limb 0 only), and for the
+// interleaved kernels whether equal-weight products share accumulators and
+// the waves per SIMD the registers are budgeted for.
 enum I8Kind { I8_REG, I8_RING, I8_RING_K2, I8_HYBRID, I8_INTERLEAVED, I8_PERSISTENT };
 struct I8Variant {
     int wr, wk, stages, kind, da;
+    bool mg = false;
+    int occ = 2;
+    int kw = 1;  // key blocks per wave (block tile: wr x wk kw blocks of 32)
 };
 static const I8Variant kVariants[] = {
     {2, 2, 0, I8_REG, 4},         {1, 4, 0, I8_REG, 4},         {2, 4, 0, I8_REG, 4},
@@ -853,6 +922,16 @@ static const I8Variant kVariants[] = {
     {2, 4, 4, I8_INTERLEAVED, 3}, {1, 4, 3, I8_INTERLEAVED, 4}, {1, 4, 3, I8_INTERLEAVED, 3},
     {2, 2, 4, I8_PERSISTENT, 4},  {2, 4, 4, I8_PERSISTENT, 4},  {2, 2, 4, I8_PERSISTENT, 3},
     {2, 4, 4, I8_PERSISTENT, 3},
+    {2, 2, 4, I8_INTERLEAVED, 4, true, 3}, {1, 4, 4, I8_INTERLEAVED, 4, true, 3},
+    {2, 2, 4, I8_INTERLEAVED, 3, true, 3}, {1, 4, 4, I8_INTERLEAVED, 3, true, 3},
+    {2, 2, 4, I8_PERSISTENT, 4, true, 3},  {2, 2, 4, I8_PERSISTENT, 3, true, 3},
+    {2, 4, 4, I8_PERSISTENT, 4, true, 2},  {2, 4, 4, I8_PERSISTENT, 3, true, 2},
+    {2, 2, 3, I8_INTERLEAVED, 4, true, 3}, {2, 2, 3, I8_INTERLEAVED, 3, true, 4},
+    {1, 4, 3, I8_INTERLEAVED, 3, true, 4},
+    {4, 1, 3, I8_INTERLEAVED, 4, true, 2, 2}, {4, 1, 3, I8_INTERLEAVED, 3, true, 2, 2},
+    {2, 2, 4, I8_INTERLEAVED, 4, true, 2, 2}, {2, 2, 4, I8_INTERLEAVED, 3, true, 2, 2},
+    {2, 2, 3, I8_INTERLEAVED, 4, true, 2, 2}, {2, 2, 5, I8_INTERLEAVED, 4, true, 2, 2},
+    {2, 2, 5, I8_INTERLEAVED, 3, true, 2, 2},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -860,12 +939,13 @@ int i8_shape_count() { return kNumVariants; }
 void i8_shape(int v, int *wr, int *wk)
 {
     *wr = kVariants[v].wr;
-    *wk = kVariants[v].wk;
+    *wk = kVariants[v].wk * kVariants[v].kw;
 }
 int i8_variant_stages(int v) { return kVariants[v].stages; }
 // "WRxWK" for the register-staged kernel; "WRxWK/STAGES" plus "k2" (two k
-// steps per barrier), "h" (hybrid), "i" (interleaved; "i3": three row
-// digits) or "ip" (interleaved, persistent; "ip3") for the others
+// steps per barrier), "h" (hybrid), "i" (interleaved) or "ip" (interleaved,
+// persistent) for the others; the interleaved ones add "m" for merged
+// accumulators, "w" for waves of two key blocks and "3" for three row digits
 const char *i8_variant_label(int v)
 {
     struct Labels {
@@ -876,13 +956,14 @@ const char *i8_variant_label(int v)
                 const I8Variant &x = kVariants[i];
                 const char *suffix = x.kind == I8_RING_K2       ? "k2"
                                      : x.kind == I8_HYBRID      ? "h"
-                                     : x.kind == I8_INTERLEAVED ? (x.da == 3 ? "i3" : "i")
-                                     : x.kind == I8_PERSISTENT  ? (x.da == 3 ? "ip3" : "ip")
+                                     : x.kind == I8_INTERLEAVED ? "i"
+                                     : x.kind == I8_PERSISTENT  ? "ip"
                                                                 : "";
                 if (x.kind == I8_REG)
                     snprintf(s[i], sizeof s[i], "%dx%d", x.wr, x.wk);
                 else
-                    snprintf(s[i], sizeof s[i], "%dx%d/%d%s", x.wr, x.wk, x.stages, suffix);
+                    snprintf(s[i], sizeof s[i], "%dx%d/%d%s%s%s%s", x.wr, x.wk * x.kw, x.stages, suffix,
+                             x.mg ? "m" : "", x.kw > 1 ? "w" : "", x.da == 3 ? "3" : "");
             }
         }
     };
@@ -985,7 +1066,7 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
                        variant, nb);
         return -1;
     }
-    const int WR = kVariants[variant].wr, WK = kVariants[variant].wk;
+    const int WR = kVariants[variant].wr, WK = kVariants[variant].wk * kVariants[variant].kw;
     // row tiles never reach past the 128 padding rows; key tiles stay within
     // the 128-key padding
     const int key_tile = 32 * WK, row_tile = 32 * WR;
@@ -995,7 +1076,7 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
     }
     const int ntk = (S + key_tile - 1) / key_tile, ntr = (R + row_tile - 1) / row_tile;
     const int ntiles = ntk * ntr;
-    const dim3 grid(((ntiles + 7) / 8) * 8), block(64 * WR * WK);
+    const dim3 grid(((ntiles + 7) / 8) * 8), block(64 * WR * kVariants[variant].wk);
     const v4i *a = reinterpret_cast<const v4i *>(A);
     const v4i *b = reinterpret_cast<const v4i *>(B);
     const long long as = (long long)ksteps_a * DG_A * FRAG_V4, bs = (long long)ksteps * nb * FRAG_V4;
@@ -1010,14 +1091,15 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 #define CI8D(wr, wk, n, stg, kpb)                                                                                  \
     hipLaunchKernelGGL((k_contract_i8_dma<wr, wk, n, stg, kpb>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, \
                        ntk, ntiles)
-#define CI8I(wr, wk, n, stg, da)                                                                                     \
-    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, false>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, \
-                       q, D, ntk, ntiles)
-#define CI8P(wr, wk, n, stg, da)                                                                                     \
-    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, true>),                                                 \
-                       dim3(persistent_grid(reinterpret_cast<const void *>(&k_contract_i8_il<wr, wk, n, stg, da, true>), \
-                                            block.x, ntiles)),                                                      \
-                       block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
+#define CI8I(wr, wk, n, stg, da, mg, occ, kw)                                                                        \
+    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, false, mg, occ, kw>), grid, block, 0, st, a, b, R, S,     \
+                       ksteps, as, bs, q, D, ntk, ntiles)
+#define CI8P(wr, wk, n, stg, da, mg, occ, kw)                                                                        \
+    hipLaunchKernelGGL(                                                                                              \
+        (k_contract_i8_il<wr, wk, n, stg, da, true, mg, occ, kw>),                                                   \
+        dim3(persistent_grid(reinterpret_cast<const void *>(&k_contract_i8_il<wr, wk, n, stg, da, true, mg, occ, kw>), \
+                             block.x, ntiles)),                                                                      \
+        block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
 #define CI8H(wk, n, stg)                                                                                             \
     hipLaunchKernelGGL((k_contract_i8_hyb<wk, n, stg>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk,    \
                        ntiles)
@@ -1034,20 +1116,38 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         case 8: CI8D(1, 4, 2, 2, 2); break;
         case 9: CI8H(4, 2, 3); break;
         case 10: CI8H(4, 2, 2); break;
-        case 11: CI8I(2, 2, 2, 3, 4); break;
-        case 12: CI8I(2, 2, 2, 4, 4); break;
-        case 13: CI8I(1, 4, 2, 4, 4); break;
-        case 14: CI8I(2, 4, 2, 4, 4); break;
-        case 15: CI8I(2, 2, 2, 3, 3); break;
-        case 16: CI8I(2, 2, 2, 4, 3); break;
-        case 17: CI8I(1, 4, 2, 4, 3); break;
-        case 18: CI8I(2, 4, 2, 4, 3); break;
-        case 19: CI8I(1, 4, 2, 3, 4); break;
-        case 20: CI8I(1, 4, 2, 3, 3); break;
-        case 21: CI8P(2, 2, 2, 4, 4); break;
-        case 22: CI8P(2, 4, 2, 4, 4); break;
-        case 23: CI8P(2, 2, 2, 4, 3); break;
-        default: CI8P(2, 4, 2, 4, 3); break;  // variant 24
+        case 11: CI8I(2, 2, 2, 3, 4, false, 2, 1); break;
+        case 12: CI8I(2, 2, 2, 4, 4, false, 2, 1); break;
+        case 13: CI8I(1, 4, 2, 4, 4, false, 2, 1); break;
+        case 14: CI8I(2, 4, 2, 4, 4, false, 2, 1); break;
+        case 15: CI8I(2, 2, 2, 3, 3, false, 2, 1); break;
+        case 16: CI8I(2, 2, 2, 4, 3, false, 2, 1); break;
+        case 17: CI8I(1, 4, 2, 4, 3, false, 2, 1); break;
+        case 18: CI8I(2, 4, 2, 4, 3, false, 2, 1); break;
+        case 19: CI8I(1, 4, 2, 3, 4, false, 2, 1); break;
+        case 20: CI8I(1, 4, 2, 3, 3, false, 2, 1); break;
+        case 21: CI8P(2, 2, 2, 4, 4, false, 2, 1); break;
+        case 22: CI8P(2, 4, 2, 4, 4, false, 2, 1); break;
+        case 23: CI8P(2, 2, 2, 4, 3, false, 2, 1); break;
+        case 24: CI8P(2, 4, 2, 4, 3, false, 2, 1); break;
+        case 25: CI8I(2, 2, 2, 4, 4, true, 3, 1); break;
+        case 26: CI8I(1, 4, 2, 4, 4, true, 3, 1); break;
+        case 27: CI8I(2, 2, 2, 4, 3, true, 3, 1); break;
+        case 28: CI8I(1, 4, 2, 4, 3, true, 3, 1); break;
+        case 29: CI8P(2, 2, 2, 4, 4, true, 3, 1); break;
+        case 30: CI8P(2, 2, 2, 4, 3, true, 3, 1); break;
+        case 31: CI8P(2, 4, 2, 4, 4, true, 2, 1); break;
+        case 32: CI8P(2, 4, 2, 4, 3, true, 2, 1); break;
+        case 33: CI8I(2, 2, 2, 3, 4, true, 3, 1); break;
+        case 34: CI8I(2, 2, 2, 3, 3, true, 4, 1); break;
+        case 35: CI8I(1, 4, 2, 3, 3, true, 4, 1); break;
+        case 36: CI8I(4, 1, 2, 3, 4, true, 2, 2); break;
+        case 37: CI8I(4, 1, 2, 3, 3, true, 2, 2); break;
+        case 38: CI8I(2, 2, 2, 4, 4, true, 2, 2); break;
+        case 39: CI8I(2, 2, 2, 4, 3, true, 2, 2); break;
+        case 40: CI8I(2, 2, 2, 3, 4, true, 2, 2); break;
+        case 41: CI8I(2, 2, 2, 5, 4, true, 2, 2); break;
+        default: CI8I(2, 2, 2, 5, 3, true, 2, 2); break;  // variant 42
         }
     } else {
         switch (variant) {

@@ -783,12 +783,11 @@ extern "C" int cufe_ctx_model_info(const cufe_ctx *c, int *kb, unsigned int *max
 // Number of int8 contraction variants, and whether variant v runs with nb key
 // digits on limb 0 (every variant valid on limbs 1-2 also runs on limb 0;
 // CUFE_I8_VARIANT=v forces one on the limbs it runs on).
-extern "C" int cufe_contract_variants(int nb, unsigned int *valid_mask)
+extern "C" int cufe_contract_variants(int nb, unsigned long long *valid_mask)
 {
-    static_assert(sizeof(unsigned int) * 8 >= 32, "mask of 32 variants");
-    unsigned int m = 0;
-    for (int v = 0; v < i8_shape_count() && v < 32; ++v)
-        if (i8_variant_ok(v, nb, 0)) m |= 1u << v;
+    unsigned long long m = 0;
+    for (int v = 0; v < i8_shape_count() && v < 64; ++v)
+        if (i8_variant_ok(v, nb, 0)) m |= 1ull << v;
     if (valid_mask) *valid_mask = m;
     return i8_shape_count();
 }

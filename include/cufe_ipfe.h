@@ -209,7 +209,7 @@ int cufe_ctx_contract_digits(const cufe_ctx *ctx);
 /* Number of int8 contraction variants (block shapes and pipeline depths);
  * valid_mask bit v = variant v runs with nb key digits.  CUFE_I8_VARIANT=v
  * forces variant v instead of the tuned choice. */
-int cufe_contract_variants(int nb, unsigned int *valid_mask);
+int cufe_contract_variants(int nb, unsigned long long *valid_mask);
 int cufe_ctx_get_row(const cufe_ctx *ctx, int row, uint32_t *cl, uint32_t *c0);
 /* Stored rows [row0, row0+n) in compact wire form: c_L [n][3][4096] and
  * coefficient 0 of c_0..c_{l-1} as [n][l][3] (what a serving request carries). */

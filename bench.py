@@ -23,6 +23,7 @@ import ctypes as C
 import hashlib
 import json
 import os
+import re
 import sys
 import time
 
@@ -452,7 +453,8 @@ def main():
         return r
 
     roof = roofline([1, 2], 4)
-    roof["limb0"] = roofline([0], 3 if i8 and tuned.get("limb 0", "").endswith("i3") else 4)
+    # labels of the kernels that use three row digits end in a letter and "3"
+    roof["limb0"] = roofline([0], 3 if i8 and re.search(r"/\d+[a-z]+3$", tuned.get("limb 0", "")) else 4)
     roof["modular_macs_per_s"] = 3.0 * R * S * K * args.steps / (contract_ms * 1e-3)
     out = {
         "metric": "IPFE-SVM predictions/sec on 1 MI355X, batched vs plain (bit-exact); p99 ms",
@@ -499,6 +501,7 @@ def main():
         log("cpu baseline (oracle, all granted cores)")
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(cf, ctx, model, x[0], args.cpu_seconds / 2)
     ctx.close()
+    out["autotune"] = cf.autotune_report().strip().split("\n")  # every shape tuned, serving batches included
     print(json.dumps(out))
 
 

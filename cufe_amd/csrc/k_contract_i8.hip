@@ -346,12 +346,13 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 // queued MFMAs instead of between steps, and a second register set holds the
 // next step's fragments.
 //
-// MG: the products of equal weight 256^(d + e) add into one accumulator
+// The products of equal weight 256^(d + e) add into one accumulator
 // (DA + NB - 1 of them instead of DA NB), which frees the registers for
 // KW = 2: each wave then computes a 32 x 64 output block from 4 key
 // fragments and DA row fragments per step -- a third fewer LDS reads per
 // MFMA than 32 x 32 -- and, at two 4-wave blocks per CU, one block's
-// epilogue runs while the other's MFMAs keep the matrix pipe busy.
+// epilogue runs while the other's MFMAs keep the matrix pipe busy.  OCC is
+// the waves per SIMD the registers are budgeted for.
 //
 // DA = 3 uses only row digits 0..2: valid when every row value lies in
 // [-8421504, 8355711] (digit 3 is then zero), which the row digit store
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 // The main loop runs while the ring still has steps to load; the last
 // STAGES + 1 steps (at most) wait with the conservative vmcnt(0).
 // ---------------------------------------------------------------------------
-template <int WR, int WK, int NB, int STAGES, int DA, bool PT, bool MG, int OCC, int KW>
+template <int WR, int WK, int NB, int STAGES, int DA, bool PT, int OCC, int KW>
 __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i *__restrict__ A,
                                                                     const v4i *__restrict__ B, int R, int S,
                                                                     int ksteps, long long a_stride,
@@ -372,7 +373,6 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
     constexpr int F = WR * DG_A + WK * KW * NB;
     static_assert(F % W == 0, "fragments must split evenly over the waves");
     static_assert(DA * NB >= 5 && DA <= DG_A, "the interleave needs five MFMAs per step");
-    static_assert(KW == 1 || MG, "a wave of KW > 1 key blocks needs the merged accumulators' registers");
     constexpr int FW = F / W;
     constexpr int STEP_V4 = F * FRAG_V4;
     constexpr int NM = DA * NB * KW;  // MFMAs per step
@@ -436,9 +436,8 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
     };
     const int a_off = wr * DG_A * FRAG_V4 + lane;
     const int b_off = (WR * DG_A + wk * KW * NB) * FRAG_V4 + lane;
-    // per key block of the wave, MG: one accumulator per weight 256^(d + e);
-    // else one per (d, e)
-    constexpr int NACC = MG ? DA + NB - 1 : DA * NB;
+    // per key block of the wave, one accumulator per weight 256^(d + e)
+    constexpr int NACC = DA + NB - 1;
     v16i acc[KW * NACC];
 #pragma unroll
     for (int c = 0; c < KW * NACC; ++c) acc[c] = v16i{};
@@ -468,8 +467,8 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
             // (reversed on odd d): two consecutive MFMAs never add into the
             // same accumulator
             const int kb = i % KW, j = i / KW;
-            const int d = j / NB, e = (MG && (d & 1)) ? NB - 1 - j % NB : j % NB;
-            const int c = kb * NACC + (MG ? d + e : j);
+            const int d = j / NB, e = (d & 1) ? NB - 1 - j % NB : j % NB;
+            const int c = kb * NACC + d + e;
             acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d], fb[kb * NB + e], acc[c], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
             if (i == 1) {
@@ -503,11 +502,7 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
         for (int kb = 0; kb < KW; ++kb) {
             const int key = tk * (32 * WK * KW) + (wk * KW + kb) * 32 + (lane & 31);
             if (key >= S) continue;
-            if constexpr (MG)
-                i8_store_merged<DA, NB>(reinterpret_cast<const v16i(&)[NACC]>(acc[kb * NACC]), row0, key, R, S, q,
-                                        out);
-            else
-                i8_store<DA, NB>(reinterpret_cast<const v16i(&)[DA][NB]>(acc[kb * NACC]), row0, key, R, S, q, out);
+            i8_store_merged<DA, NB>(reinterpret_cast<const v16i(&)[NACC]>(acc[kb * NACC]), row0, key, R, S, q, out);
         }
     };
     // after step g of a persistent block: at the end of a tile, store it and
@@ -554,7 +549,7 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
 // CRT and rounding.  Two key digits, four row digits on every limb.
 // ---------------------------------------------------------------------------
 template <int WR, int WK, int STAGES>
-__global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_split(
+__global__ __launch_bounds__(64 * WR * WK, 3) void k_contract_i8_split(
     const v4i *__restrict__ A, long long a_limb, const v4i *__restrict__ B, long long b_limb, int R, int S, int ksteps,
     long long a_stride, long long b_stride, int nsplit, int kper, int ntk, int ntiles, uint32_t *__restrict__ Dpart)
 {
@@ -603,11 +598,10 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_split(
     };
     const int a_off = wr * DG_A * FRAG_V4 + lane;
     const int b_off = (WR * DG_A + wk * NB) * FRAG_V4 + lane;
-    v16i acc[DA][NB];
+    constexpr int NACC = DA + NB - 1;  // one accumulator per weight 256^(d + e)
+    v16i acc[NACC];
 #pragma unroll
-    for (int d = 0; d < DA; ++d)
-#pragma unroll
-        for (int e = 0; e < NB; ++e) acc[d][e] = v16i{};
+    for (int c = 0; c < NACC; ++c) acc[c] = v16i{};
 #pragma unroll
     for (int p = 0; p < STAGES; ++p)
         if (p < G) issue_next();
@@ -629,8 +623,8 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_split(
         const v4i *nx = ring + rd_slot * STEP_V4;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            acc[i / NB][i % NB] =
-                __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i / NB], fb[i % NB], acc[i / NB][i % NB], 0, 0, 0);
+            const int d = i / NB, e = (d & 1) ? NB - 1 - i % NB : i % NB;
+            acc[d + e] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d], fb[e], acc[d + e], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
             if (i == 1) {
                 if constexpr (M)
@@ -669,7 +663,7 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_split(
     const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
     if (key >= S) return;
     const I8Out out = {Dpart + (size_t)(s * nsplit + slice) * R * S, nullptr, nullptr, nullptr};
-    i8_store<DA, NB>(acc, tr * (32 * WR) + wr * 32 + (lane >> 5) * 4, key, R, S, qmod(s), out);
+    i8_store_merged<DA, NB>(acc, tr * (32 * WR) + wr * 32 + (lane >> 5) * 4, key, R, S, qmod(s), out);
 }
 
 // Sums the slices of k_contract_i8_split mod q per limb, then CRT and rounding.
@@ -903,35 +897,29 @@ int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int
 // Kernel variants the launcher can run: row blocks x key blocks per block
 // tile, the LDS ring depth (0: the register-staged double-buffered kernel),
 // the kernel, the row digits the MFMAs use (3: limb 0 only), and for the
-// interleaved kernels whether equal-weight products share accumulators and
-// the waves per SIMD the registers are budgeted for.
+// interleaved kernels the waves per SIMD the registers are budgeted for and
+// the key blocks per wave (block tile: wr x wk kw blocks of 32).
 enum I8Kind { I8_REG, I8_RING, I8_RING_K2, I8_HYBRID, I8_INTERLEAVED, I8_PERSISTENT };
 struct I8Variant {
     int wr, wk, stages, kind, da;
-    bool mg = false;
     int occ = 2;
-    int kw = 1;  // key blocks per wave (block tile: wr x wk kw blocks of 32)
+    int kw = 1;
 };
 static const I8Variant kVariants[] = {
-    {2, 2, 0, I8_REG, 4},         {1, 4, 0, I8_REG, 4},         {2, 4, 0, I8_REG, 4},
-    {2, 2, 4, I8_RING, 4},        {1, 4, 4, I8_RING, 4},        {2, 2, 3, I8_RING, 4},
-    {1, 4, 3, I8_RING, 4},        {2, 4, 2, I8_RING_K2, 4},     {1, 4, 2, I8_RING_K2, 4},
-    {1, 4, 3, I8_HYBRID, 4},      {1, 4, 2, I8_HYBRID, 4},      {2, 2, 3, I8_INTERLEAVED, 4},
-    {2, 2, 4, I8_INTERLEAVED, 4}, {1, 4, 4, I8_INTERLEAVED, 4}, {2, 4, 4, I8_INTERLEAVED, 4},
-    {2, 2, 3, I8_INTERLEAVED, 3}, {2, 2, 4, I8_INTERLEAVED, 3}, {1, 4, 4, I8_INTERLEAVED, 3},
-    {2, 4, 4, I8_INTERLEAVED, 3}, {1, 4, 3, I8_INTERLEAVED, 4}, {1, 4, 3, I8_INTERLEAVED, 3},
-    {2, 2, 4, I8_PERSISTENT, 4},  {2, 4, 4, I8_PERSISTENT, 4},  {2, 2, 4, I8_PERSISTENT, 3},
-    {2, 4, 4, I8_PERSISTENT, 3},
-    {2, 2, 4, I8_INTERLEAVED, 4, true, 3}, {1, 4, 4, I8_INTERLEAVED, 4, true, 3},
-    {2, 2, 4, I8_INTERLEAVED, 3, true, 3}, {1, 4, 4, I8_INTERLEAVED, 3, true, 3},
-    {2, 2, 4, I8_PERSISTENT, 4, true, 3},  {2, 2, 4, I8_PERSISTENT, 3, true, 3},
-    {2, 4, 4, I8_PERSISTENT, 4, true, 2},  {2, 4, 4, I8_PERSISTENT, 3, true, 2},
-    {2, 2, 3, I8_INTERLEAVED, 4, true, 3}, {2, 2, 3, I8_INTERLEAVED, 3, true, 4},
-    {1, 4, 3, I8_INTERLEAVED, 3, true, 4},
-    {4, 1, 3, I8_INTERLEAVED, 4, true, 2, 2}, {4, 1, 3, I8_INTERLEAVED, 3, true, 2, 2},
-    {2, 2, 4, I8_INTERLEAVED, 4, true, 2, 2}, {2, 2, 4, I8_INTERLEAVED, 3, true, 2, 2},
-    {2, 2, 3, I8_INTERLEAVED, 4, true, 2, 2}, {2, 2, 5, I8_INTERLEAVED, 4, true, 2, 2},
-    {2, 2, 5, I8_INTERLEAVED, 3, true, 2, 2},
+    // register-staged, LDS rings and the hybrid (also the three-key-digit set)
+    {2, 2, 0, I8_REG, 4}, {1, 4, 0, I8_REG, 4}, {2, 4, 0, I8_REG, 4}, {1, 4, 4, I8_RING, 4}, {1, 4, 3, I8_RING, 4},
+    {1, 4, 2, I8_RING_K2, 4}, {1, 4, 3, I8_HYBRID, 4},
+    // interleaved, merged accumulators, 32 x 32 per wave
+    {2, 2, 4, I8_INTERLEAVED, 4, 3}, {1, 4, 4, I8_INTERLEAVED, 4, 3}, {2, 2, 4, I8_INTERLEAVED, 3, 3},
+    {1, 4, 4, I8_INTERLEAVED, 3, 3}, {2, 2, 3, I8_INTERLEAVED, 4, 3}, {2, 2, 3, I8_INTERLEAVED, 3, 4},
+    {1, 4, 3, I8_INTERLEAVED, 3, 4},
+    // the same, persistent
+    {2, 2, 4, I8_PERSISTENT, 4, 3}, {2, 2, 4, I8_PERSISTENT, 3, 3}, {2, 4, 4, I8_PERSISTENT, 4, 2},
+    {2, 4, 4, I8_PERSISTENT, 3, 2},
+    // interleaved, 32 x 64 per wave
+    {4, 1, 3, I8_INTERLEAVED, 4, 2, 2}, {4, 1, 3, I8_INTERLEAVED, 3, 2, 2}, {2, 2, 4, I8_INTERLEAVED, 4, 2, 2},
+    {2, 2, 4, I8_INTERLEAVED, 3, 2, 2}, {2, 2, 3, I8_INTERLEAVED, 4, 2, 2}, {2, 2, 5, I8_INTERLEAVED, 4, 2, 2},
+    {2, 2, 5, I8_INTERLEAVED, 3, 2, 2},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -944,8 +932,8 @@ void i8_shape(int v, int *wr, int *wk)
 int i8_variant_stages(int v) { return kVariants[v].stages; }
 // "WRxWK" for the register-staged kernel; "WRxWK/STAGES" plus "k2" (two k
 // steps per barrier), "h" (hybrid), "i" (interleaved) or "ip" (interleaved,
-// persistent) for the others; the interleaved ones add "m" for merged
-// accumulators, "w" for waves of two key blocks and "3" for three row digits
+// persistent, each with "m": merged accumulators) for the others; "w" for
+// waves of two key blocks and "3" for three row digits
 const char *i8_variant_label(int v)
 {
     struct Labels {
@@ -956,14 +944,14 @@ const char *i8_variant_label(int v)
                 const I8Variant &x = kVariants[i];
                 const char *suffix = x.kind == I8_RING_K2       ? "k2"
                                      : x.kind == I8_HYBRID      ? "h"
-                                     : x.kind == I8_INTERLEAVED ? "i"
-                                     : x.kind == I8_PERSISTENT  ? "ip"
+                                     : x.kind == I8_INTERLEAVED ? "im"
+                                     : x.kind == I8_PERSISTENT  ? "ipm"
                                                                 : "";
                 if (x.kind == I8_REG)
                     snprintf(s[i], sizeof s[i], "%dx%d", x.wr, x.wk);
                 else
-                    snprintf(s[i], sizeof s[i], "%dx%d/%d%s%s%s%s", x.wr, x.wk * x.kw, x.stages, suffix,
-                             x.mg ? "m" : "", x.kw > 1 ? "w" : "", x.da == 3 ? "3" : "");
+                    snprintf(s[i], sizeof s[i], "%dx%d/%d%s%s%s", x.wr, x.wk * x.kw, x.stages, suffix,
+                             x.kw > 1 ? "w" : "", x.da == 3 ? "3" : "");
             }
         }
     };
@@ -1091,13 +1079,13 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 #define CI8D(wr, wk, n, stg, kpb)                                                                                  \
     hipLaunchKernelGGL((k_contract_i8_dma<wr, wk, n, stg, kpb>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, \
                        ntk, ntiles)
-#define CI8I(wr, wk, n, stg, da, mg, occ, kw)                                                                        \
-    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, false, mg, occ, kw>), grid, block, 0, st, a, b, R, S,     \
-                       ksteps, as, bs, q, D, ntk, ntiles)
-#define CI8P(wr, wk, n, stg, da, mg, occ, kw)                                                                        \
+#define CI8I(wr, wk, n, stg, da, occ, kw)                                                                            \
+    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, false, occ, kw>), grid, block, 0, st, a, b, R, S, ksteps, \
+                       as, bs, q, D, ntk, ntiles)
+#define CI8P(wr, wk, n, stg, da, occ, kw)                                                                            \
     hipLaunchKernelGGL(                                                                                              \
-        (k_contract_i8_il<wr, wk, n, stg, da, true, mg, occ, kw>),                                                   \
-        dim3(persistent_grid(reinterpret_cast<const void *>(&k_contract_i8_il<wr, wk, n, stg, da, true, mg, occ, kw>), \
+        (k_contract_i8_il<wr, wk, n, stg, da, true, occ, kw>),                                                       \
+        dim3(persistent_grid(reinterpret_cast<const void *>(&k_contract_i8_il<wr, wk, n, stg, da, true, occ, kw>),   \
                              block.x, ntiles)),                                                                      \
         block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
 #define CI8H(wk, n, stg)                                                                                             \
@@ -1108,56 +1096,38 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         case 0: CI8(2, 2, 2); break;
         case 1: CI8(1, 4, 2); break;
         case 2: CI8(2, 4, 2); break;
-        case 3: CI8D(2, 2, 2, 4, 1); break;
-        case 4: CI8D(1, 4, 2, 4, 1); break;
-        case 5: CI8D(2, 2, 2, 3, 1); break;
-        case 6: CI8D(1, 4, 2, 3, 1); break;
-        case 7: CI8D(2, 4, 2, 2, 2); break;
-        case 8: CI8D(1, 4, 2, 2, 2); break;
-        case 9: CI8H(4, 2, 3); break;
-        case 10: CI8H(4, 2, 2); break;
-        case 11: CI8I(2, 2, 2, 3, 4, false, 2, 1); break;
-        case 12: CI8I(2, 2, 2, 4, 4, false, 2, 1); break;
-        case 13: CI8I(1, 4, 2, 4, 4, false, 2, 1); break;
-        case 14: CI8I(2, 4, 2, 4, 4, false, 2, 1); break;
-        case 15: CI8I(2, 2, 2, 3, 3, false, 2, 1); break;
-        case 16: CI8I(2, 2, 2, 4, 3, false, 2, 1); break;
-        case 17: CI8I(1, 4, 2, 4, 3, false, 2, 1); break;
-        case 18: CI8I(2, 4, 2, 4, 3, false, 2, 1); break;
-        case 19: CI8I(1, 4, 2, 3, 4, false, 2, 1); break;
-        case 20: CI8I(1, 4, 2, 3, 3, false, 2, 1); break;
-        case 21: CI8P(2, 2, 2, 4, 4, false, 2, 1); break;
-        case 22: CI8P(2, 4, 2, 4, 4, false, 2, 1); break;
-        case 23: CI8P(2, 2, 2, 4, 3, false, 2, 1); break;
-        case 24: CI8P(2, 4, 2, 4, 3, false, 2, 1); break;
-        case 25: CI8I(2, 2, 2, 4, 4, true, 3, 1); break;
-        case 26: CI8I(1, 4, 2, 4, 4, true, 3, 1); break;
-        case 27: CI8I(2, 2, 2, 4, 3, true, 3, 1); break;
-        case 28: CI8I(1, 4, 2, 4, 3, true, 3, 1); break;
-        case 29: CI8P(2, 2, 2, 4, 4, true, 3, 1); break;
-        case 30: CI8P(2, 2, 2, 4, 3, true, 3, 1); break;
-        case 31: CI8P(2, 4, 2, 4, 4, true, 2, 1); break;
-        case 32: CI8P(2, 4, 2, 4, 3, true, 2, 1); break;
-        case 33: CI8I(2, 2, 2, 3, 4, true, 3, 1); break;
-        case 34: CI8I(2, 2, 2, 3, 3, true, 4, 1); break;
-        case 35: CI8I(1, 4, 2, 3, 3, true, 4, 1); break;
-        case 36: CI8I(4, 1, 2, 3, 4, true, 2, 2); break;
-        case 37: CI8I(4, 1, 2, 3, 3, true, 2, 2); break;
-        case 38: CI8I(2, 2, 2, 4, 4, true, 2, 2); break;
-        case 39: CI8I(2, 2, 2, 4, 3, true, 2, 2); break;
-        case 40: CI8I(2, 2, 2, 3, 4, true, 2, 2); break;
-        case 41: CI8I(2, 2, 2, 5, 4, true, 2, 2); break;
-        default: CI8I(2, 2, 2, 5, 3, true, 2, 2); break;  // variant 42
+        case 3: CI8D(1, 4, 2, 4, 1); break;
+        case 4: CI8D(1, 4, 2, 3, 1); break;
+        case 5: CI8D(1, 4, 2, 2, 2); break;
+        case 6: CI8H(4, 2, 3); break;
+        case 7: CI8I(2, 2, 2, 4, 4, 3, 1); break;
+        case 8: CI8I(1, 4, 2, 4, 4, 3, 1); break;
+        case 9: CI8I(2, 2, 2, 4, 3, 3, 1); break;
+        case 10: CI8I(1, 4, 2, 4, 3, 3, 1); break;
+        case 11: CI8I(2, 2, 2, 3, 4, 3, 1); break;
+        case 12: CI8I(2, 2, 2, 3, 3, 4, 1); break;
+        case 13: CI8I(1, 4, 2, 3, 3, 4, 1); break;
+        case 14: CI8P(2, 2, 2, 4, 4, 3, 1); break;
+        case 15: CI8P(2, 2, 2, 4, 3, 3, 1); break;
+        case 16: CI8P(2, 4, 2, 4, 4, 2, 1); break;
+        case 17: CI8P(2, 4, 2, 4, 3, 2, 1); break;
+        case 18: CI8I(4, 1, 2, 3, 4, 2, 2); break;
+        case 19: CI8I(4, 1, 2, 3, 3, 2, 2); break;
+        case 20: CI8I(2, 2, 2, 4, 4, 2, 2); break;
+        case 21: CI8I(2, 2, 2, 4, 3, 2, 2); break;
+        case 22: CI8I(2, 2, 2, 3, 4, 2, 2); break;
+        case 23: CI8I(2, 2, 2, 5, 4, 2, 2); break;
+        default: CI8I(2, 2, 2, 5, 3, 2, 2); break;  // variant 24
         }
     } else {
         switch (variant) {
         case 0: CI8(2, 2, 3); break;
         case 1: CI8(1, 4, 3); break;
         case 2: CI8(2, 4, 3); break;
-        case 4: CI8D(1, 4, 3, 4, 1); break;
-        case 6: CI8D(1, 4, 3, 3, 1); break;
-        case 8: CI8D(1, 4, 3, 2, 2); break;
-        default: CI8H(4, 3, 3); break;  // variant 9
+        case 3: CI8D(1, 4, 3, 4, 1); break;
+        case 4: CI8D(1, 4, 3, 3, 1); break;
+        case 5: CI8D(1, 4, 3, 2, 2); break;
+        default: CI8H(4, 3, 3); break;  // variant 6
         }
     }
 #undef CI8

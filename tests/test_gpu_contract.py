@@ -82,6 +82,39 @@ def test_kernels_agree(gpu, hi, digits):
     ctx.close()
 
 
+def test_longest_rows_every_variant(gpu):
+    """Rows of all 785 slots and keys whose support reaches slot 783 give the
+    longest contraction (K = 4,880 columns, 153 k steps): every int8 variant,
+    the split-K form and the fp64 kernel agree, and every key value decrypts
+    <x, y> mod p."""
+    rng = np.random.default_rng(34)
+    l, R, S = 785, 200, 150
+    ctx = gpu.Context()
+    ctx.setup(h(b"wA"), h(b"wS"), h(b"wE"), 785)
+    x = rng.integers(0, 5, (R, l)).astype(np.uint32)
+    ctx.rows_reserve(R, l)
+    ctx.encrypt_rows(x, h(b"w2"), h(b"w3"), nonce0=0)
+    sv = np.zeros((S, 785), np.uint32)
+    for v in range(S):
+        sv[v, rng.choice(np.arange(1, 784), 12, replace=False)] = 1
+    sv[0, 783] = 1
+    coef = rng.integers(-16, 17, (1, S)).astype(np.float64)
+    model = gpu.SvmModel(2, [S // 2, S - S // 2], [1, -1], coef, [0.5], sv)
+    ctx.set_model(model)
+    assert ctx.contract_digits() == 2
+    lab0, kv0 = predict_with(ctx, R, {})
+    ip = (x.astype(np.int64) @ sv.astype(np.int64).T) % 50241
+    assert np.array_equal(kv0.astype(np.int64) % 50241, ip)
+    for env in [{"CUFE_I8_VARIANT": str(v)} for v in gpu.contract_variants(2)] + [{"CUFE_CONTRACT": "sgpr"}]:
+        lab, kv = predict_with(ctx, R, env)
+        assert np.array_equal(kv, kv0), env
+        assert np.array_equal(lab, lab0), env
+    lab_s, kv_s, _ = ctx.predict(10, 100, want_kvalues=True)  # split-K: at most 256 rows
+    assert np.array_equal(kv_s, kv0[10:110]) and np.array_equal(lab_s, lab0[10:110])
+    note("785-slot rows: every variant equal")
+    ctx.close()
+
+
 def test_split_k_small_batches_agree(gpu):
     """Row ranges of at most 256 rows run all limbs and slices of the k steps
     as one split-K launch plus a finish kernel (sum of slices mod q, CRT);

@@ -960,8 +960,8 @@ const char *i8_variant_label(int v)
 }
 // Variants instantiated for nb key digits on limb s (three row digits only on
 // limb 0, whose row digit store holds a zero digit 3).  With three key digits:
-// the register-staged kernels and the 1x4 tiles of the others (the 1x4/3
-// hybrid only).
+// the register-staged kernels and the non-persistent 1x4 tiles of the others
+// (the 1x4/3 hybrid only), whose ring slots split evenly over the waves.
 bool i8_variant_ok(int v, int nb, int s)
 {
     if (v < 0 || v >= kNumVariants || nb < 2 || nb > 3) return false;
@@ -969,7 +969,7 @@ bool i8_variant_ok(int v, int nb, int s)
     if (x.da == 3 && s != 0) return false;
     if (nb == 2 || x.kind == I8_REG) return true;
     if (x.kind == I8_HYBRID) return x.stages == 3;
-    return x.wr == 1 && x.kind != I8_INTERLEAVED && x.kind != I8_PERSISTENT;
+    return x.wr == 1 && x.kw == 1 && x.kind != I8_PERSISTENT;
 }
 
 // Grid of a persistent kernel: every CU filled once at the kernel's
@@ -1127,7 +1127,10 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         case 3: CI8D(1, 4, 3, 4, 1); break;
         case 4: CI8D(1, 4, 3, 3, 1); break;
         case 5: CI8D(1, 4, 3, 2, 2); break;
-        default: CI8H(4, 3, 3); break;  // variant 6
+        case 6: CI8H(4, 3, 3); break;
+        case 8: CI8I(1, 4, 3, 4, 4, 2, 1); break;
+        case 10: CI8I(1, 4, 3, 4, 3, 2, 1); break;
+        default: CI8I(1, 4, 3, 3, 3, 2, 1); break;  // variant 13
         }
     }
 #undef CI8

@@ -929,7 +929,6 @@ void i8_shape(int v, int *wr, int *wk)
     *wr = kVariants[v].wr;
     *wk = kVariants[v].wk * kVariants[v].kw;
 }
-int i8_variant_stages(int v) { return kVariants[v].stages; }
 // "WRxWK" for the register-staged kernel; "WRxWK/STAGES" plus "k2" (two k
 // steps per barrier), "h" (hybrid), "i" (interleaved) or "ip" (interleaved,
 // persistent, each with "m": merged accumulators) for the others; "w" for

@@ -45,7 +45,6 @@ int launch_contract_i8_split(const int8_t *A, size_t a_limb_bytes, int ksteps_a,
                              hipStream_t st);
 int i8_shape_count();
 void i8_shape(int variant, int *wr, int *wk);
-int i8_variant_stages(int variant);
 const char *i8_variant_label(int variant);
 bool i8_variant_ok(int variant, int nb, int s);
 // largest key coefficients the two- and three-digit key splits hold: the top

@@ -101,4 +101,74 @@ __device__ __forceinline__ void aes256_ctr_block_rep(const uint32_t *__restrict_
     o[3] = __builtin_bswap32(f3);
 }
 
+// Counter-mode shortcut for the bank-replicated tables.  The counter sits in
+// state word 3, so in round 1 only the lookup of its low byte changes between
+// the 256 counters that share bits 8..31: t0 = C0 ^ ror(T(s3 & 255), 24), and
+// t1..t3 are constant.  In round 2 each output word then has one lookup that
+// depends on t0 and three that do not.  aes256_ctr_window computes, once per
+// such window of counters, w = {C0, t1, t2, t3, D0..D3} with D_j the constant
+// part of round-2 word j; aes256_ctr_block_win finishes a block from it with
+// 5 lookups in rounds 1-2 instead of 32.  The output is that of
+// aes256_ctr_block_rep for the same counter.
+__device__ __forceinline__ void aes256_ctr_window(const uint32_t *__restrict__ rk, const uint32_t *te_rep,
+                                                  uint32_t nonce, uint32_t ctr_hi, uint32_t *w)
+{
+    const uint32_t *T = te_rep + (threadIdx.x & 31);
+#define TL(x) T[(x) << 5]
+    const uint32_t s0 = (nonce << 16) ^ rk[0], s1 = rk[1], s2 = rk[2], s3 = (ctr_hi << 8) ^ rk[3];
+    const uint32_t c0 = TL(s0 >> 24) ^ ror32(TL((s1 >> 16) & 255), 8) ^ ror32(TL((s2 >> 8) & 255), 16) ^ rk[4];
+    const uint32_t t1 = TL(s1 >> 24) ^ ror32(TL((s2 >> 16) & 255), 8) ^ ror32(TL((s3 >> 8) & 255), 16) ^
+                        ror32(TL(s0 & 255), 24) ^ rk[5];
+    const uint32_t t2 = TL(s2 >> 24) ^ ror32(TL((s3 >> 16) & 255), 8) ^ ror32(TL((s0 >> 8) & 255), 16) ^
+                        ror32(TL(s1 & 255), 24) ^ rk[6];
+    const uint32_t t3 = TL(s3 >> 24) ^ ror32(TL((s0 >> 16) & 255), 8) ^ ror32(TL((s1 >> 8) & 255), 16) ^
+                        ror32(TL(s2 & 255), 24) ^ rk[7];
+    w[0] = c0;
+    w[1] = t1;
+    w[2] = t2;
+    w[3] = t3;
+    w[4] = ror32(TL((t1 >> 16) & 255), 8) ^ ror32(TL((t2 >> 8) & 255), 16) ^ ror32(TL(t3 & 255), 24) ^ rk[8];
+    w[5] = TL(t1 >> 24) ^ ror32(TL((t2 >> 16) & 255), 8) ^ ror32(TL((t3 >> 8) & 255), 16) ^ rk[9];
+    w[6] = TL(t2 >> 24) ^ ror32(TL((t3 >> 16) & 255), 8) ^ ror32(TL(t1 & 255), 24) ^ rk[10];
+    w[7] = TL(t3 >> 24) ^ ror32(TL((t1 >> 8) & 255), 16) ^ ror32(TL(t2 & 255), 24) ^ rk[11];
+#undef TL
+}
+
+__device__ __forceinline__ void aes256_ctr_block_win(const uint32_t *__restrict__ rk, const uint32_t *te_rep,
+                                                     const uint32_t *w, uint32_t ctr, uint32_t (&o)[4])
+{
+    const uint32_t *T = te_rep + (threadIdx.x & 31);
+#define TL(x) T[(x) << 5]
+    const uint32_t t0 = w[0] ^ ror32(TL((ctr ^ rk[3]) & 255), 24);
+    uint32_t s0 = TL(t0 >> 24) ^ w[4];
+    uint32_t s1 = ror32(TL(t0 & 255), 24) ^ w[5];
+    uint32_t s2 = ror32(TL((t0 >> 8) & 255), 16) ^ w[6];
+    uint32_t s3 = ror32(TL((t0 >> 16) & 255), 8) ^ w[7];
+#pragma unroll
+    for (int r = 3; r < 14; ++r) {
+        uint32_t t0 = TL(s0 >> 24) ^ ror32(TL((s1 >> 16) & 255), 8) ^ ror32(TL((s2 >> 8) & 255), 16) ^
+                      ror32(TL(s3 & 255), 24) ^ rk[4 * r];
+        uint32_t t1 = TL(s1 >> 24) ^ ror32(TL((s2 >> 16) & 255), 8) ^ ror32(TL((s3 >> 8) & 255), 16) ^
+                      ror32(TL(s0 & 255), 24) ^ rk[4 * r + 1];
+        uint32_t t2 = TL(s2 >> 24) ^ ror32(TL((s3 >> 16) & 255), 8) ^ ror32(TL((s0 >> 8) & 255), 16) ^
+                      ror32(TL(s1 & 255), 24) ^ rk[4 * r + 2];
+        uint32_t t3 = TL(s3 >> 24) ^ ror32(TL((s0 >> 16) & 255), 8) ^ ror32(TL((s1 >> 8) & 255), 16) ^
+                      ror32(TL(s2 & 255), 24) ^ rk[4 * r + 3];
+        s0 = t0; s1 = t1; s2 = t2; s3 = t3;
+    }
+    uint32_t f0 = ((TL(s0 >> 24) << 16) & 0xff000000u) ^ (TL((s1 >> 16) & 255) & 0x00ff0000u) ^
+                  (TL((s2 >> 8) & 255) & 0x0000ff00u) ^ ((TL(s3 & 255) >> 8) & 0xffu) ^ rk[56];
+    uint32_t f1 = ((TL(s1 >> 24) << 16) & 0xff000000u) ^ (TL((s2 >> 16) & 255) & 0x00ff0000u) ^
+                  (TL((s3 >> 8) & 255) & 0x0000ff00u) ^ ((TL(s0 & 255) >> 8) & 0xffu) ^ rk[57];
+    uint32_t f2 = ((TL(s2 >> 24) << 16) & 0xff000000u) ^ (TL((s3 >> 16) & 255) & 0x00ff0000u) ^
+                  (TL((s0 >> 8) & 255) & 0x0000ff00u) ^ ((TL(s1 & 255) >> 8) & 0xffu) ^ rk[58];
+    uint32_t f3 = ((TL(s3 >> 24) << 16) & 0xff000000u) ^ (TL((s0 >> 16) & 255) & 0x00ff0000u) ^
+                  (TL((s1 >> 8) & 255) & 0x0000ff00u) ^ ((TL(s2 & 255) >> 8) & 0xffu) ^ rk[59];
+#undef TL
+    o[0] = __builtin_bswap32(f0);
+    o[1] = __builtin_bswap32(f1);
+    o[2] = __builtin_bswap32(f2);
+    o[3] = __builtin_bswap32(f3);
+}
+
 }  // namespace ipfe

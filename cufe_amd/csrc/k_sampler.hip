@@ -183,15 +183,19 @@ __device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int wh
 // nb: blocks per chunk -- 24 (all), PART_BLOCKS (0..15 and 17) or 16 (0..15,
 // for streams whose values are not stored: the sign byte then matters only for
 // a zero candidate, and that block is computed where it is needed).
+// win (bank-replicated tables only; may be null): the counter windows
+// w0, w0 + 1, ... of aes256_ctr_window, 8 words each, covering every counter.
 template <bool REP, int NT, typename ChunkOf>
 __device__ __forceinline__ void aes_chunks(const GaussStream *S, const uint32_t *te, int nch, ChunkOf chunk_of,
-                                           uint32_t *ks, int full, int nb)
+                                           uint32_t *ks, int full, int nb, const uint32_t *win, uint32_t w0)
 {
     for (int g = threadIdx.x; g < nch * nb; g += NT) {
         const int k = g / nb, blk = chunk_block(g - nb * k, full);
         const uint32_t ctr = (uint32_t)(chunk_of(k) * 24 + blk);
         uint32_t o[4];
-        if (REP)
+        if (REP && win)
+            aes256_ctr_block_win(S->rk, te, win + 8 * ((ctr >> 8) - w0), ctr, o);
+        else if (REP)
             aes256_ctr_block_rep(S->rk, te, S->nonce, ctr, o);
         else
             aes256_ctr_block(S->rk, te, S->nonce, ctr, o);
@@ -211,8 +215,11 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
                                                                     uint8_t *__restrict__ cnt, uint64_t *__restrict__ vals,
                                                                     int *__restrict__ status, int full, int ulim)
 {
+    // windows of 256 counters one tile spans (its 24 TILE_CHUNKS counters)
+    constexpr int WINDOWS = TILE_CHUNKS * 24 / 256 + 2;
     __shared__ uint32_t te[REP ? AES_REP_WORDS : AES_TABLE_WORDS];
     __shared__ uint32_t ks[TILE_CHUNKS * CHUNK_WORDS];
+    __shared__ uint32_t win[REP ? WINDOWS * 8 : 8];
     if (REP)
         aes_stage_tables_rep(te, te_global);
     else
@@ -228,9 +235,16 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
         // without stored values only a zero candidate needs its sign (a zero
         // with sign 0 is rejected, src/gauss.c:1086-1163)
         const bool lazy_sign = !store_vals && !full;
-        __syncthreads();  // tables staged / previous tile's reads of ks done
+        __syncthreads();  // tables staged / previous tile's reads of ks and win done
+        // counter windows of the tile (unless its counters wrap at 2^32)
+        const uint32_t c_lo = (uint32_t)(chunk0 * 24), w0 = c_lo >> 8;
+        const bool use_win = REP && chunk0 * 24 + (long long)nch * 24 <= (1ll << 32);
+        const int nw = (int)((c_lo + (uint32_t)(nch * 24 - 1)) >> 8) - (int)w0 + 1;
+        if (use_win && (int)threadIdx.x < nw)
+            aes256_ctr_window(S->rk, te, S->nonce, w0 + threadIdx.x, win + 8 * threadIdx.x);
+        __syncthreads();
         aes_chunks<REP, NT>(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks, full,
-                            full ? 24 : lazy_sign ? 16 : PART_BLOCKS);
+                            full ? 24 : lazy_sign ? 16 : PART_BLOCKS, use_win ? win : nullptr, w0);
         __syncthreads();
         for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += NT) {
             const int k = slot >> 3, c = slot & 7;

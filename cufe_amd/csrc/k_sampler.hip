@@ -15,9 +15,10 @@
 //      (call, first output position);
 //   3. scatter: every accepted candidate of a used chunk writes its three
 //      residues to its call's polynomial.
-#include <algorithm>
-
 #include <stdlib.h>
+
+#include <algorithm>
+#include <type_traits>
 
 #include "aes_dev.h"
 #include "kernels.h"
@@ -183,9 +184,27 @@ __device__ __forceinline__ Cand eval_candidate(const uint32_t *ch, int c, int wh
 // nb: blocks per chunk -- 24 (all), PART_BLOCKS (0..15 and 17) or 16 (0..15,
 // for streams whose values are not stored: the sign byte then matters only for
 // a zero candidate, and that block is computed where it is needed).
-// win (bank-replicated tables only; may be null): the counter windows
-// w0, w0 + 1, ... of aes256_ctr_window, 8 words each, covering every counter.
-template <bool REP, int NT, typename ChunkOf>
+// TM: the T-table layout in te -- 0: four 1 KiB tables; 1: T0 replicated per
+// bank (AesTabRep); 2: T0 and T2 replicated per bank (AesTabRep2).
+// win (TM 1 and 2; may be null): the counter windows w0, w0 + 1, ... of
+// aes256_ctr_window, 8 words each, covering every counter.
+template <int TM>
+__device__ __forceinline__ void aes_block(const GaussStream *S, const uint32_t *te, const uint32_t *win, uint32_t w0,
+                                          uint32_t ctr, uint32_t (&o)[4])
+{
+    if constexpr (TM == 0) {
+        aes256_ctr_block(S->rk, te, S->nonce, ctr, o);
+    } else {
+        using Tab = std::conditional_t<TM == 2, AesTabRep2, AesTabRep>;
+        const Tab tb(te);
+        if (win)
+            aes256_ctr_block_win(tb, S->rk, win + 8 * ((ctr >> 8) - w0), ctr, o);
+        else
+            aes256_ctr_block_t(tb, S->rk, S->nonce, ctr, o);
+    }
+}
+
+template <int TM, int NT, typename ChunkOf>
 __device__ __forceinline__ void aes_chunks(const GaussStream *S, const uint32_t *te, int nch, ChunkOf chunk_of,
                                            uint32_t *ks, int full, int nb, const uint32_t *win, uint32_t w0)
 {
@@ -193,21 +212,17 @@ __device__ __forceinline__ void aes_chunks(const GaussStream *S, const uint32_t 
         const int k = g / nb, blk = chunk_block(g - nb * k, full);
         const uint32_t ctr = (uint32_t)(chunk_of(k) * 24 + blk);
         uint32_t o[4];
-        if (REP && win)
-            aes256_ctr_block_win(S->rk, te, win + 8 * ((ctr >> 8) - w0), ctr, o);
-        else if (REP)
-            aes256_ctr_block_rep(S->rk, te, S->nonce, ctr, o);
-        else
-            aes256_ctr_block(S->rk, te, S->nonce, ctr, o);
+        aes_block<TM>(S, te, win, w0, ctr, o);
         uint32_t *d = ks + k * CHUNK_WORDS + blk * 4;
         d[0] = o[0]; d[1] = o[1]; d[2] = o[2]; d[3] = o[3];
     }
 }
 
-// REP: bank-replicated T-table (32 KiB of LDS) instead of the four 1 KiB
-// tables, shared by NT threads (512 with the replicated table, so that two
-// workgroups per CU still hold 16 waves).
-template <bool REP, int NT>
+// TM (aes_block): the four 1 KiB T-tables, T0 replicated per bank (32 KiB of
+// LDS; NT = 512, so that two workgroups per CU hold 16 waves) or T0 and T2
+// replicated (64 KiB; the keystream of a tile then runs in two passes of half
+// a tile, so that two workgroups still fit a CU's LDS).
+template <int TM, int NT>
 __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__restrict__ streams, int nstreams,
                                                                     const long long *__restrict__ tile_prefix,
                                                                     long long ntiles,
@@ -217,10 +232,15 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
 {
     // windows of 256 counters one tile spans (its 24 TILE_CHUNKS counters)
     constexpr int WINDOWS = TILE_CHUNKS * 24 / 256 + 2;
-    __shared__ uint32_t te[REP ? AES_REP_WORDS : AES_TABLE_WORDS];
-    __shared__ uint32_t ks[TILE_CHUNKS * CHUNK_WORDS];
-    __shared__ uint32_t win[REP ? WINDOWS * 8 : 8];
-    if (REP)
+    // chunks per keystream pass: with the 64 KiB tables, half a tile, so that
+    // two workgroups fit a CU's LDS
+    constexpr int SUB = TM == 2 ? TILE_CHUNKS / 2 : TILE_CHUNKS;
+    __shared__ uint32_t te[TM == 2 ? AES_REP2_WORDS : TM == 1 ? AES_REP_WORDS : AES_TABLE_WORDS];
+    __shared__ uint32_t ks[SUB * CHUNK_WORDS];
+    __shared__ uint32_t win[TM ? WINDOWS * 8 : 8];
+    if (TM == 2)
+        aes_stage_tables_rep2(te, te_global);
+    else if (TM == 1)
         aes_stage_tables_rep(te, te_global);
     else
         aes_stage_tables(te, te_global);
@@ -238,38 +258,41 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
         __syncthreads();  // tables staged / previous tile's reads of ks and win done
         // counter windows of the tile (unless its counters wrap at 2^32)
         const uint32_t c_lo = (uint32_t)(chunk0 * 24), w0 = c_lo >> 8;
-        const bool use_win = REP && chunk0 * 24 + (long long)nch * 24 <= (1ll << 32);
+        const bool use_win = TM && chunk0 * 24 + (long long)nch * 24 <= (1ll << 32);
         const int nw = (int)((c_lo + (uint32_t)(nch * 24 - 1)) >> 8) - (int)w0 + 1;
-        if (use_win && (int)threadIdx.x < nw)
-            aes256_ctr_window(S->rk, te, S->nonce, w0 + threadIdx.x, win + 8 * threadIdx.x);
-        __syncthreads();
-        aes_chunks<REP, NT>(S, te, nch, [chunk0](int k) { return chunk0 + k; }, ks, full,
-                            full ? 24 : lazy_sign ? 16 : PART_BLOCKS, use_win ? win : nullptr, w0);
-        __syncthreads();
-        for (int slot = threadIdx.x; slot < TILE_CHUNKS * 8; slot += NT) {
-            const int k = slot >> 3, c = slot & 7;
-            const bool valid = k < nch;
-            // with lazy_sign, block 17 of chunk k is computed by the asking lane
-            // alone (a zero candidate: about 1e-9 of S2/S3 candidates)
-            auto sign_word = [&](const uint32_t *ch) -> uint32_t {
-                if (!lazy_sign) return ch[70];
-                uint32_t o[4];
-                const uint32_t ctr = (uint32_t)((chunk0 + k) * 24 + 17);
-                if (REP)
-                    aes256_ctr_block_rep(S->rk, te, S->nonce, ctr, o);
-                else
-                    aes256_ctr_block(S->rk, te, S->nonce, ctr, o);
-                return o[2];
-            };
-            const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status, full, ulim, sign_word,
-                                           lazy_sign);
-            const uint64_t am = __ballot(cd.acc);
-            const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
-            if (valid) {
-                const long long gc = S->chunk_base + chunk0 + k;
-                if (c == 0) cnt[gc] = (uint8_t)__popc(gm);
-                if (store_vals && cd.acc)
-                    vals[gc * 8 + __popc(gm & ((1u << c) - 1u))] = cd.z | ((uint64_t)cd.sgn << 63);
+        if constexpr (TM != 0) {
+            using Tab = std::conditional_t<TM == 2, AesTabRep2, AesTabRep>;
+            if (use_win && (int)threadIdx.x < nw)
+                aes256_ctr_window(Tab(te), S->rk, S->nonce, w0 + threadIdx.x, win + 8 * threadIdx.x);
+        }
+        for (int h = 0; h < nch; h += SUB) {
+            __syncthreads();  // windows written / the previous pass's reads of ks done
+            const long long c0 = chunk0 + h;
+            const int nsub = min(SUB, nch - h);
+            aes_chunks<TM, NT>(S, te, nsub, [c0](int k) { return c0 + k; }, ks, full,
+                               full ? 24 : lazy_sign ? 16 : PART_BLOCKS, use_win ? win : nullptr, w0);
+            __syncthreads();
+            for (int slot = threadIdx.x; slot < SUB * 8; slot += NT) {
+                const int k = slot >> 3, c = slot & 7;
+                const bool valid = k < nsub;
+                // with lazy_sign, block 17 of chunk k is computed by the asking
+                // lane alone (a zero candidate: about 1e-9 of S2/S3 candidates)
+                auto sign_word = [&](const uint32_t *ch) -> uint32_t {
+                    if (!lazy_sign) return ch[70];
+                    uint32_t o[4];
+                    aes_block<TM>(S, te, nullptr, 0, (uint32_t)((c0 + k) * 24 + 17), o);
+                    return o[2];
+                };
+                const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status, full, ulim, sign_word,
+                                               lazy_sign);
+                const uint64_t am = __ballot(cd.acc);
+                const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
+                if (valid) {
+                    const long long gc = S->chunk_base + c0 + k;
+                    if (c == 0) cnt[gc] = (uint8_t)__popc(gm);
+                    if (store_vals && cd.acc)
+                        vals[gc * 8 + __popc(gm & ((1u << c) - 1u))] = cd.z | ((uint64_t)cd.sgn << 63);
+                }
             }
         }
     }
@@ -494,12 +517,13 @@ __global__ void k_aes_ctr(const uint32_t *__restrict__ rk, const uint32_t *__res
 }
 
 // ---------------------------------------------------------------------------
-// 0: four T-tables, 256 threads; 1 (default): replicated table, 512 threads;
-// 2: replicated table, 256 threads (CUFE_AES_REP)
+// 0: four T-tables, 256 threads; 1: T0 replicated, 512 threads; 2: T0
+// replicated, 256 threads; 3 (default): T0 and T2 replicated, 512 threads,
+// keystream in half-tile passes (CUFE_AES_REP)
 static int aes_rep_mode()
 {
     const char *e = getenv("CUFE_AES_REP");
-    return e && *e ? atoi(e) : 1;
+    return e && *e ? atoi(e) : 3;
 }
 
 // Uniform words a partial chunk may scan before it asks for all 24 blocks:
@@ -518,14 +542,17 @@ int launch_gauss_candidates(const GaussStream *d_streams, int nstreams, const lo
     if (ntiles <= 0) return 0;
     const unsigned grid = (unsigned)std::min<long long>(ntiles, 1ll << 20);
     const int rep = aes_rep_mode();
-    if (rep == 2)
-        hipLaunchKernelGGL((k_gauss_candidates<true, 256>), dim3(grid), dim3(256), 0, st, d_streams, nstreams,
+    if (rep >= 3)
+        hipLaunchKernelGGL((k_gauss_candidates<2, 512>), dim3(grid), dim3(512), 0, st, d_streams, nstreams,
                            d_tile_prefix, ntiles, g_aes_te, cnt, vals, status, full, scan_limit());
-    else if (rep)
-        hipLaunchKernelGGL((k_gauss_candidates<true, 512>), dim3(grid), dim3(512), 0, st, d_streams, nstreams,
+    else if (rep == 2)
+        hipLaunchKernelGGL((k_gauss_candidates<1, 256>), dim3(grid), dim3(256), 0, st, d_streams, nstreams,
+                           d_tile_prefix, ntiles, g_aes_te, cnt, vals, status, full, scan_limit());
+    else if (rep == 1)
+        hipLaunchKernelGGL((k_gauss_candidates<1, 512>), dim3(grid), dim3(512), 0, st, d_streams, nstreams,
                            d_tile_prefix, ntiles, g_aes_te, cnt, vals, status, full, scan_limit());
     else
-        hipLaunchKernelGGL((k_gauss_candidates<false, CAND_THREADS>), dim3(grid), dim3(CAND_THREADS), 0, st,
+        hipLaunchKernelGGL((k_gauss_candidates<0, CAND_THREADS>), dim3(grid), dim3(CAND_THREADS), 0, st,
                            d_streams, nstreams, d_tile_prefix, ntiles, g_aes_te, cnt, vals, status, full,
                            scan_limit());
     HIP_OK(hipGetLastError());

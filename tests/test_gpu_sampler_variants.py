@@ -1,5 +1,6 @@
 """The sampler's AES and rerun variants give the oracle's samples: the
-bank-replicated and the four-table T-table AES, and a lowered partial-chunk
+two-table (default) and one-table bank-replicated layouts and the four-table
+T-table AES, and a lowered partial-chunk
 scan limit that makes many chunks ask for all 24 AES blocks (the rerun path
 that natural data reaches about once in 1e11 chunks).  Checked on multi-call
 streams (full outputs), on compact streams (sample 0 of each call, against the
@@ -12,8 +13,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [{"CUFE_AES_REP": "0"}, {"CUFE_AES_REP": "1"}, {"CUFE_GAUSS_SCAN": "8"},
-            {"CUFE_GAUSS_SCAN": "8", "CUFE_AES_REP": "0"}]
+VARIANTS = [{"CUFE_AES_REP": "0"}, {"CUFE_AES_REP": "1"}, {"CUFE_AES_REP": "2"}, {"CUFE_GAUSS_SCAN": "8"},
+            {"CUFE_GAUSS_SCAN": "8", "CUFE_AES_REP": "1"}]
 
 
 def h(t: bytes) -> bytes:
@@ -51,7 +52,7 @@ def test_stream_variants_match_oracle(gpu, po, which, variant):
         assert np.array_equal(got[t], po.gauss_calls(keys[t], 5 + t, which, 2)), (t, variant)
 
 
-@pytest.mark.parametrize("variant", [{}, {"CUFE_AES_REP": "0"}, {"CUFE_GAUSS_SCAN": "8"}])
+@pytest.mark.parametrize("variant", [{}, {"CUFE_AES_REP": "0"}, {"CUFE_AES_REP": "1"}, {"CUFE_GAUSS_SCAN": "8"}])
 @pytest.mark.parametrize("which", [1, 2, 3])
 def test_first_samples_equal_full_streams(gpu, which, variant):
     """Compact streams compute the sign block of a chunk only for a zero

@@ -222,7 +222,7 @@ def microbench(kind: int) -> float:
     """0: fp64 FMA TFLOP/s, 1: int32 mul T ops/s, 2: copy GB/s, 3: AES G blocks/s,
     4: AES (replicated table) G blocks/s, 5: int8 MFMA T ops/s, 6: fp64 MFMA TFLOP/s,
     7: NTT butterflies in registers G/s, 8: int8 MFMA T ops/s on random operands,
-    sustained over 8 back-to-back launches."""
+    sustained over 8 back-to-back launches, 9: AES (T0 and T2 replicated) G blocks/s."""
     r = C.c_double()
     _check(lib().cufe_microbench(kind, C.byref(r)), "microbench")
     return r.value

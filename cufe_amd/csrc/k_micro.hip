@@ -92,6 +92,25 @@ __global__ __launch_bounds__(256) void k_mb_aes_rep(const uint32_t *__restrict__
     out[blockIdx.x * blockDim.x + threadIdx.x] = x;
 }
 
+// T0 and T2 bank-replicated (64 KiB): the sampler's default AES.
+__global__ __launch_bounds__(256) void k_mb_aes_rep2(const uint32_t *__restrict__ rk,
+                                                     const uint32_t *__restrict__ te_global,
+                                                     uint32_t nblocks_per_thread, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t te[AES_REP2_WORDS];
+    aes_stage_tables_rep2(te, te_global);
+    __syncthreads();
+    const AesTabRep2 tb(te);
+    uint32_t x = 0;
+    const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) * nblocks_per_thread;
+    for (uint32_t i = 0; i < nblocks_per_thread; ++i) {
+        uint32_t o[4];
+        aes256_ctr_block_t(tb, rk, 7u, base + i, o);
+        x ^= o[0] ^ o[1] ^ o[2] ^ o[3];
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
 typedef int mb_v4i __attribute__((ext_vector_type(4)));
 typedef int mb_v16i __attribute__((ext_vector_type(16)));
 typedef double mb_v4d __attribute__((ext_vector_type(4)));
@@ -168,20 +187,39 @@ __global__ __launch_bounds__(256) void k_mb_bfly(uint32_t *out, int iters, uint3
     out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
-// Checks that both AES variants produce the same keystream words.
-__global__ void k_mb_aes_check(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global,
+// Checks that the AES variants produce the keystream words of the four-table
+// AES: the replicated T0 table (rep2 = 0) or T0 and T2 (rep2 = 1), whole
+// blocks and counter windows (the window of the block's counter is computed by
+// the same lane).
+__global__ void k_mb_aes_check(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global, int rep2,
                                uint32_t *__restrict__ bad)
 {
     __shared__ uint32_t te[AES_TABLE_WORDS];
-    __shared__ uint32_t rep[AES_REP_WORDS];
+    __shared__ uint32_t rep[AES_REP2_WORDS];
     aes_stage_tables(te, te_global);
-    aes_stage_tables_rep(rep, te_global);
+    if (rep2)
+        aes_stage_tables_rep2(rep, te_global);
+    else
+        aes_stage_tables_rep(rep, te_global);
     __syncthreads();
-    uint32_t o1[4], o2[4];
+    uint32_t o1[4], o2[4], o3[4], w[8];
     const uint32_t ctr = blockIdx.x * blockDim.x + threadIdx.x;
     aes256_ctr_block(rk, te, 9u, ctr, o1);
-    aes256_ctr_block_rep(rk, rep, 9u, ctr, o2);
-    if (o1[0] != o2[0] || o1[1] != o2[1] || o1[2] != o2[2] || o1[3] != o2[3]) atomicAdd(bad, 1u);
+    if (rep2) {
+        const AesTabRep2 tb(rep);
+        aes256_ctr_block_t(tb, rk, 9u, ctr, o2);
+        aes256_ctr_window(tb, rk, 9u, ctr >> 8, w);
+        aes256_ctr_block_win(tb, rk, w, ctr, o3);
+    } else {
+        const AesTabRep tb(rep);
+        aes256_ctr_block_rep(rk, rep, 9u, ctr, o2);
+        aes256_ctr_window(tb, rk, 9u, ctr >> 8, w);
+        aes256_ctr_block_win(tb, rk, w, ctr, o3);
+    }
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ok = ok && o1[j] == o2[j] && o1[j] == o3[j];
+    if (!ok) atomicAdd(bad, 1u);
 }
 
 }  // namespace ipfe
@@ -192,7 +230,9 @@ using namespace ipfe;
 // 2: device copy (GB/s of read+write), 3: AES-256 CTR blocks (G blocks/s),
 // 4: the same with the bank-replicated table (and an equality check),
 // 5: int8 MFMA 32x32x32 (T ops/s), 6: fp64 MFMA 16x16x4 (TFLOP/s),
-// 7: NTT butterflies in registers (G butterflies/s).
+// 7: NTT butterflies in registers (G butterflies/s), 8: int8 MFMA on random
+// operands (T ops/s), 9: AES with T0 and T2 replicated (G blocks/s, and an
+// equality check).
 extern "C" int cufe_microbench(int kind, double *result)
 {
     Device *dev = device();
@@ -265,7 +305,7 @@ extern "C" int cufe_microbench(int kind, double *result)
         HIP_OK(hipEventElapsedTime(&ms, a, b));
         const double waves = (double)blocks * 4;
         work = kind == 5 ? waves * iters * 2 * (32.0 * 32 * 32 * 2) : waves * iters * 4 * (16.0 * 16 * 4 * 2);
-    } else if (kind == 0 || kind == 1 || kind == 3 || kind == 4) {
+    } else if (kind == 0 || kind == 1 || kind == 3 || kind == 4 || kind == 9) {
         DevBuf<uint32_t> out;
         if (out.alloc((size_t)blocks * 256 * 2)) return -1;
         const int iters = 4096;
@@ -285,8 +325,10 @@ extern "C" int cufe_microbench(int kind, double *result)
                 hipLaunchKernelGGL(k_mb_mul32, dim3(blocks), dim3(256), 0, st, out.get(), iters, 2654435761u);
             else if (kind == 3)
                 hipLaunchKernelGGL(k_mb_aes, dim3(blocks), dim3(256), 0, st, rk.get(), g_aes_te, 64u, out.get());
-            else
+            else if (kind == 4)
                 hipLaunchKernelGGL(k_mb_aes_rep, dim3(blocks), dim3(256), 0, st, rk.get(), g_aes_te, 64u, out.get());
+            else
+                hipLaunchKernelGGL(k_mb_aes_rep2, dim3(blocks), dim3(256), 0, st, rk.get(), g_aes_te, 64u, out.get());
             HIP_OK(hipGetLastError());
             HIP_OK(hipEventRecord(b, st));
             HIP_OK(hipEventSynchronize(b));
@@ -296,12 +338,13 @@ extern "C" int cufe_microbench(int kind, double *result)
         if (kind == 0) work = threads * iters * 16 * 2;       // FLOPs
         else if (kind == 1) work = threads * iters * 8 * 3;   // int ops
         else work = threads * 64;                             // AES blocks
-        if (kind == 4) {
+        if (kind == 4 || kind == 9) {
             DevBuf<uint32_t> bad;
             uint32_t hb = 0;
             if (bad.alloc(1)) return -1;
             HIP_OK(hipMemset(bad.get(), 0, 4));
-            hipLaunchKernelGGL(k_mb_aes_check, dim3(64), dim3(256), 0, st, rk.get(), g_aes_te, bad.get());
+            hipLaunchKernelGGL(k_mb_aes_check, dim3(64), dim3(256), 0, st, rk.get(), g_aes_te, kind == 9 ? 1 : 0,
+                               bad.get());
             HIP_OK(hipMemcpy(&hb, bad.get(), 4, hipMemcpyDeviceToHost));
             if (hb) {
                 ipfe_set_error("bank-replicated AES disagrees with the T-table AES on %u blocks", hb);

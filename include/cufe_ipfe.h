@@ -128,7 +128,8 @@ int cufe_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r);
  * 3 = AES-256-CTR keystream (G blocks/s), 4 = the same with the
  * bank-replicated table, 5 = int8 MFMA 32x32x32 (T ops/s), 6 = fp64 MFMA
  * 16x16x4 (TFLOP/s), 7 = NTT butterflies in registers (G/s), 8 = int8 MFMA
- * on random operands, sustained over 8 back-to-back launches (T ops/s). */
+ * on random operands, sustained over 8 back-to-back launches (T ops/s), 9 =
+ * AES-256-CTR with T0 and T2 replicated per bank (G blocks/s). */
 int cufe_microbench(int kind, double *result);
 
 /* The scheme at a vector length l <= 785 (uneven-length batching).  With

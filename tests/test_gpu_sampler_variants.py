@@ -88,3 +88,14 @@ def test_compact_rows_variants_agree(gpu):
         else:
             assert np.array_equal(cl, ref[0]) and np.array_equal(c0, ref[1]), variant
     ctx.close()
+
+
+def test_aes_table_layouts_agree(gpu):
+    """The bank-replicated T0 table and the T0 + T2 tables, for whole blocks and
+    through counter windows, give the four-table AES's blocks (the check runs
+    inside the microbenchmark over 16,384 counters and fails it on a
+    difference)."""
+    for kind in (4, 9):
+        rate = gpu.microbench(kind)
+        print(f"[aes] microbench({kind}): {rate:.1f} G blocks/s", flush=True)
+        assert rate > 0

@@ -265,26 +265,21 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
             if (use_win && (int)threadIdx.x < nw)
                 aes256_ctr_window(Tab(te), S->rk, S->nonce, w0 + threadIdx.x, win + 8 * threadIdx.x);
         }
+        // The two-table kernel's partial chunks of stored streams: the pass
+        // computes blocks 0..15 only (SUB 16 = NT blocks, one per thread); the
+        // SUB 8 evaluating lanes take every sign as 1 while the next SUB threads,
+        // otherwise idle, compute block 17 -- the sign word -- of each chunk,
+        // and after a barrier the evaluating lanes settle sign and acceptance.
+        const bool split_sign = TM == 2 && !full && store_vals;
         for (int h = 0; h < nch; h += SUB) {
             __syncthreads();  // windows written / the previous pass's reads of ks done
             const long long c0 = chunk0 + h;
             const int nsub = min(SUB, nch - h);
             aes_chunks<TM, NT>(S, te, nsub, [c0](int k) { return c0 + k; }, ks, full,
-                               full ? 24 : lazy_sign ? 16 : PART_BLOCKS, use_win ? win : nullptr, w0);
+                               full ? 24 : (lazy_sign || split_sign) ? 16 : PART_BLOCKS, use_win ? win : nullptr, w0);
             __syncthreads();
-            for (int slot = threadIdx.x; slot < SUB * 8; slot += NT) {
-                const int k = slot >> 3, c = slot & 7;
-                const bool valid = k < nsub;
-                // with lazy_sign, block 17 of chunk k is computed by the asking
-                // lane alone (a zero candidate: about 1e-9 of S2/S3 candidates)
-                auto sign_word = [&](const uint32_t *ch) -> uint32_t {
-                    if (!lazy_sign) return ch[70];
-                    uint32_t o[4];
-                    aes_block<TM>(S, te, nullptr, 0, (uint32_t)((c0 + k) * 24 + 17), o);
-                    return o[2];
-                };
-                const Cand cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, valid, status, full, ulim, sign_word,
-                                               lazy_sign);
+            // ballot of the accept bits and the chunk's count and values
+            auto commit = [&](const Cand &cd, int k, int c, bool valid) {
                 const uint64_t am = __ballot(cd.acc);
                 const uint32_t gm = (uint32_t)((am >> sh) & 0xffu);
                 if (valid) {
@@ -293,6 +288,43 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
                     if (store_vals && cd.acc)
                         vals[gc * 8 + __popc(gm & ((1u << c) - 1u))] = cd.z | ((uint64_t)cd.sgn << 63);
                 }
+            };
+            if (split_sign) {
+                static_assert(TM != 2 || NT >= SUB * 9, "the sign blocks need SUB threads beside the SUB 8 lanes");
+                const int t = threadIdx.x, k = t >> 3, c = t & 7;
+                const bool eval = t < SUB * 8;
+                Cand cd{};
+                if (eval) {
+                    cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, k < nsub, status, full, ulim,
+                                        [](const uint32_t *) { return ~0u; });
+                } else if (t < SUB * 8 + nsub) {
+                    const int kb = t - SUB * 8;
+                    uint32_t o[4];
+                    aes_block<TM>(S, te, use_win ? win : nullptr, w0, (uint32_t)((c0 + kb) * 24 + 17), o);
+                    ks[kb * CHUNK_WORDS + 70] = o[2];
+                }
+                __syncthreads();
+                if (eval) {
+                    // with every sign 1, cd.acc was the Bernoulli bit alone
+                    cd.sgn = (ks[k * CHUNK_WORDS + 70] >> c) & 1u;
+                    cd.acc = cd.acc && (cd.z != 0 || cd.sgn != 0);
+                    commit(cd, k, c, k < nsub);
+                }
+                continue;
+            }
+            for (int slot = threadIdx.x; slot < SUB * 8; slot += NT) {
+                const int k = slot >> 3, c = slot & 7;
+                // with lazy_sign, block 17 of chunk k is computed by the asking
+                // lane alone (a zero candidate: about 1e-9 of S2/S3 candidates)
+                auto sign_word = [&](const uint32_t *ch) -> uint32_t {
+                    if (!lazy_sign) return ch[70];
+                    uint32_t o[4];
+                    aes_block<TM>(S, te, nullptr, 0, (uint32_t)((c0 + k) * 24 + 17), o);
+                    return o[2];
+                };
+                commit(eval_candidate(ks + k * CHUNK_WORDS, c, which, k < nsub, status, full, ulim, sign_word,
+                                      lazy_sign),
+                       k, c, k < nsub);
             }
         }
     }

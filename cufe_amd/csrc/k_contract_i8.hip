@@ -95,7 +95,10 @@ __device__ __forceinline__ void i8_store(const v16i (&acc)[DA][NB], int row0, in
 // top by Horner steps in doubles, reduced mod q after every two steps: a
 // reduced value is below 2^31, so two steps stay below 2^48 and the first
 // (from |C| < 2^30) below 2^46 -- every value exact.
-template <int DA, int NB>
+// LB: outputs whose limb-0 and limb-1 residues are loaded together in the CRT
+// epilogue (fewer for kernels budgeted for more waves per SIMD).  With DA = 3
+// (limb 0) the epilogue never does the CRT, which runs on limb 2.
+template <int DA, int NB, int LB = 8>
 __device__ __forceinline__ void i8_store_merged(const v16i (&acc)[DA + NB - 1], int row0, int key, int R, int S,
                                                 uint32_t q, const I8Out &out)
 {
@@ -107,10 +110,7 @@ __device__ __forceinline__ void i8_store_merged(const v16i (&acc)[DA + NB - 1], 
         m -= (m >= qd) ? qd : 0.0;
         return m;
     };
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = row0 + (i >> 2) * 8 + (i & 3);
-        if (row >= R) continue;
+    auto residue = [&](int i) {
         double T = (double)acc[NS - 1][i];
         int steps = 0;
 #pragma unroll
@@ -121,12 +121,52 @@ __device__ __forceinline__ void i8_store_merged(const v16i (&acc)[DA + NB - 1], 
                 steps = 0;
             }
         }
-        const uint32_t res = (uint32_t)T;
-        const size_t idx = (size_t)row * S + key;
-        if (out.kval)
-            out.kval[idx] = crt_decode_canon(out.d0[idx], out.d1[idx], res);
-        else
-            out.D[idx] = res;
+        return (uint32_t)T;
+    };
+    auto index = [&](int i) {  // rows past R read row R - 1 and store nothing
+        const int row = min(row0 + (i >> 2) * 8 + (i & 3), R - 1);
+        return (size_t)row * S + key;
+    };
+    auto live = [&](int i) { return row0 + (i >> 2) * 8 + (i & 3) < R; };
+    if constexpr (LB == 1) {
+        // one output at a time: the fewest live registers
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (!live(i)) continue;
+            const uint32_t res = residue(i);
+            const size_t idx = index(i);
+            if (DA == 4 && out.kval)
+                out.kval[idx] = crt_decode_canon(out.d0[idx], out.d1[idx], res);
+            else
+                out.D[idx] = res;
+        }
+    } else if (DA == 4 && out.kval) {
+        // in groups of LB outputs, the 2 LB loads of limbs 0 and 1 are issued
+        // before the first is used (one memory wait per group, not per
+        // output); the scheduling barrier keeps the groups apart so that the
+        // loaded values fit the registers the k loop leaves
+        static_assert(16 % LB == 0, "groups of outputs");
+#pragma unroll
+        for (int h = 0; h < 16; h += LB) {
+            uint32_t r0[LB], r1[LB];
+#pragma unroll
+            for (int i = 0; i < LB; ++i) {
+                r0[i] = out.d0[index(h + i)];
+                r1[i] = out.d1[index(h + i)];
+            }
+#pragma unroll
+            for (int i = 0; i < LB; ++i) {
+                const uint32_t v = crt_decode_canon(r0[i], r1[i], residue(h + i));
+                if (live(h + i)) out.kval[index(h + i)] = v;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t v = residue(i);
+            if (live(i)) out.D[index(i)] = v;
+        }
     }
 }
 
@@ -502,7 +542,10 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
         for (int kb = 0; kb < KW; ++kb) {
             const int key = tk * (32 * WK * KW) + (wk * KW + kb) * 32 + (lane & 31);
             if (key >= S) continue;
-            i8_store_merged<DA, NB>(reinterpret_cast<const v16i(&)[NACC]>(acc[kb * NACC]), row0, key, R, S, q, out);
+            // a persistent block stores with the next tile's fragments live
+            constexpr int LB = OCC < 3 ? 8 : PT ? 1 : 4;
+            i8_store_merged<DA, NB, LB>(reinterpret_cast<const v16i(&)[NACC]>(acc[kb * NACC]), row0, key, R, S, q,
+                                        out);
         }
     };
     // after step g of a persistent block: at the end of a tile, store it and

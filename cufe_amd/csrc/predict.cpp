@@ -457,27 +457,34 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
         if (kval_host)
             HIP_OK(hipMemcpyAsync(kval_host + (size_t)(r0 - row0) * S, sc.kval.get(), sizeof(uint32_t) * R * S,
                                   hipMemcpyDeviceToHost, st));
-        if (tm || r0 + chunk < row0 + nrows) {
+        if (r0 + chunk < row0 + nrows) {
             // the scratch is reused by the next chunk: wait for this one
             HIP_OK(hipStreamSynchronize(st));
-            if (tm) {
-                float a = 0, b = 0, d = 0;
-                HIP_OK(hipEventElapsedTime(&a, tm->ev[0], tm->ev[1]));
-                HIP_OK(hipEventElapsedTime(&b, tm->ev[1], tm->ev[2]));
-                HIP_OK(hipEventElapsedTime(&d, tm->ev[2], tm->ev[3]));
-                tm->ms[0] += a;
-                tm->ms[1] += b;
-                tm->ms[2] += d;
-                float l0 = 0, l1 = 0, l2 = 0;
-                HIP_OK(hipEventElapsedTime(&l0, tm->ev[0], tm->ev[4]));
-                HIP_OK(hipEventElapsedTime(&l1, tm->ev[4], tm->ev[5]));
-                HIP_OK(hipEventElapsedTime(&l2, tm->ev[5], tm->ev[1]));
-                tm->limb_ms[0] += l0;
-                tm->limb_ms[1] += l1;
-                tm->limb_ms[2] += l2;
-            }
+            if (tm && predict_timers_collect(tm)) return -1;
+        } else if (tm) {
+            tm->pending = true;  // the caller waits for the stream, then collects
         }
     }
+    return 0;
+}
+
+int predict_timers_collect(PredictTimers *tm)
+{
+    float a = 0, b = 0, d = 0;
+    HIP_OK(hipEventElapsedTime(&a, tm->ev[0], tm->ev[1]));
+    HIP_OK(hipEventElapsedTime(&b, tm->ev[1], tm->ev[2]));
+    HIP_OK(hipEventElapsedTime(&d, tm->ev[2], tm->ev[3]));
+    tm->ms[0] += a;
+    tm->ms[1] += b;
+    tm->ms[2] += d;
+    float l0 = 0, l1 = 0, l2 = 0;
+    HIP_OK(hipEventElapsedTime(&l0, tm->ev[0], tm->ev[4]));
+    HIP_OK(hipEventElapsedTime(&l1, tm->ev[4], tm->ev[5]));
+    HIP_OK(hipEventElapsedTime(&l2, tm->ev[5], tm->ev[1]));
+    tm->limb_ms[0] += l0;
+    tm->limb_ms[1] += l1;
+    tm->limb_ms[2] += l2;
+    tm->pending = false;
     return 0;
 }
 
@@ -736,6 +743,7 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
     if (!c->model.fused) HIP_OK(hipMemcpyAsync(&flag, c->scratch.flag.get(), 4, hipMemcpyDeviceToHost, st));
     HIP_OK(hipEventRecord(c->ev[5], st));
     HIP_OK(hipEventSynchronize(c->ev[5]));
+    if (tm.pending && predict_timers_collect(&tm)) return -1;
     float tot = 0;
     HIP_OK(hipEventElapsedTime(&tot, c->ev[4], c->ev[5]));
     for (int s = 0; s < 3; ++s) c->limb_ms[s] = tm.limb_ms[s];

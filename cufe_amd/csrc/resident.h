@@ -88,7 +88,10 @@ struct PredictTimers {
     hipEvent_t ev[6];
     float ms[3];
     float limb_ms[3];
+    bool pending = false;  // the last chunk's events are read after the caller's wait
 };
+// Adds the event intervals of a finished chunk to ms / limb_ms.
+int predict_timers_collect(PredictTimers *tm);
 int predict_rows(const Model &m, const RowStore &rs, int r0, int R, Scratch &sc, int scratch_rows, int mode,
                  double *labels_host, uint32_t *kval_host, PredictTimers *tm, hipStream_t st);
 

@@ -378,10 +378,10 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 
 // ---------------------------------------------------------------------------
 // Pipelined variant with the step's work interleaved by hand.  Of a step's
-// DA x NB MFMAs, the first two are issued before the wait for the next ring
-// slot and the barrier; the third is followed by the ring loads of step
+// DA x NB x KW MFMAs, the first two are issued before the wait for the next
+// ring slot and the barrier; the third is followed by the ring loads of step
 // ks + STAGES, the fourth by the LDS reads of the next step's key fragments,
-// and each later one by the read of one next row fragment.  Scheduling
+// and each of the next DA by the read of one next row fragment.  Scheduling
 // barriers keep that order, so the barrier and the LDS latency fall under
 // queued MFMAs instead of between steps, and a second register set holds the
 // next step's fragments.

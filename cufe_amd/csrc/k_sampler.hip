@@ -55,7 +55,7 @@ __constant__ uint64_t c_EXP[10] = {0x3e833b70ffa2c5d4ull, 0x3eb4a480fda7e6e1ull,
 extern const uint32_t *g_aes_te;  // device T-tables, set by the context
 
 constexpr int CAND_THREADS = 256;
-constexpr int TILE_CHUNKS = 64;
+constexpr int TILE_CHUNKS = GAUSS_TILE_CHUNKS;
 constexpr int CHUNK_WORDS = 96;
 
 __device__ __forceinline__ int find_stream(const long long *prefix, int n, long long key)
@@ -204,24 +204,29 @@ __device__ __forceinline__ void aes_block(const GaussStream *S, const uint32_t *
     }
 }
 
+// cw: words per chunk in ks (CHUNK_WORDS, or fewer when only the first blocks
+// are stored).
 template <int TM, int NT, typename ChunkOf>
 __device__ __forceinline__ void aes_chunks(const GaussStream *S, const uint32_t *te, int nch, ChunkOf chunk_of,
-                                           uint32_t *ks, int full, int nb, const uint32_t *win, uint32_t w0)
+                                           uint32_t *ks, int cw, int full, int nb, const uint32_t *win, uint32_t w0)
 {
     for (int g = threadIdx.x; g < nch * nb; g += NT) {
         const int k = g / nb, blk = chunk_block(g - nb * k, full);
         const uint32_t ctr = (uint32_t)(chunk_of(k) * 24 + blk);
         uint32_t o[4];
         aes_block<TM>(S, te, win, w0, ctr, o);
-        uint32_t *d = ks + k * CHUNK_WORDS + blk * 4;
+        uint32_t *d = ks + k * cw + blk * 4;
         d[0] = o[0]; d[1] = o[1]; d[2] = o[2]; d[3] = o[3];
     }
 }
 
 // TM (aes_block): the four 1 KiB T-tables, T0 replicated per bank (32 KiB of
 // LDS; NT = 512, so that two workgroups per CU hold 16 waves) or T0 and T2
-// replicated (64 KiB; the keystream of a tile then runs in two passes of half
-// a tile, so that two workgroups still fit a CU's LDS).
+// replicated (64 KiB).  With the 64 KiB tables two workgroups fit a CU's LDS
+// when the keystream buffer holds 64 words per chunk: streams that need only
+// blocks 0..15 run a tile in one pass; the others run two passes of 32 chunks
+// (72 words per chunk for blocks 0..15 and the sign word, 96 for all 24
+// blocks).
 template <int TM, int NT>
 __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__restrict__ streams, int nstreams,
                                                                     const long long *__restrict__ tile_prefix,
@@ -232,11 +237,13 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
 {
     // windows of 256 counters one tile spans (its 24 TILE_CHUNKS counters)
     constexpr int WINDOWS = TILE_CHUNKS * 24 / 256 + 2;
-    // chunks per keystream pass: with the 64 KiB tables, half a tile, so that
-    // two workgroups fit a CU's LDS
-    constexpr int SUB = TM == 2 ? TILE_CHUNKS / 2 : TILE_CHUNKS;
+    constexpr int KS_WORDS = TM == 2 ? TILE_CHUNKS * 64 : TILE_CHUNKS * CHUNK_WORDS;
+    constexpr int HALF = 32;  // chunks per pass of the two-pass tiles
+    static_assert(TM != 2 || (HALF * CHUNK_WORDS <= KS_WORDS && 2 * HALF >= TILE_CHUNKS && NT >= HALF * 9 &&
+                              TILE_CHUNKS * 8 <= NT),
+                  "pass sizes of the two-table kernel");
     __shared__ uint32_t te[TM == 2 ? AES_REP2_WORDS : TM == 1 ? AES_REP_WORDS : AES_TABLE_WORDS];
-    __shared__ uint32_t ks[SUB * CHUNK_WORDS];
+    __shared__ uint32_t ks[KS_WORDS];
     __shared__ uint32_t win[TM ? WINDOWS * 8 : 8];
     if (TM == 2)
         aes_stage_tables_rep2(te, te_global);
@@ -265,17 +272,21 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
             if (use_win && (int)threadIdx.x < nw)
                 aes256_ctr_window(Tab(te), S->rk, S->nonce, w0 + threadIdx.x, win + 8 * threadIdx.x);
         }
-        // The two-table kernel's partial chunks of stored streams: the pass
-        // computes blocks 0..15 only (SUB 16 = NT blocks, one per thread); the
-        // SUB 8 evaluating lanes take every sign as 1 while the next SUB threads,
-        // otherwise idle, compute block 17 -- the sign word -- of each chunk,
-        // and after a barrier the evaluating lanes settle sign and acceptance.
+        // The two-table kernel's partial chunks of stored streams: a pass of
+        // HALF chunks computes blocks 0..15 only (HALF 16 = 512 blocks); the
+        // HALF 8 evaluating lanes take every sign as 1 while the next HALF
+        // threads, otherwise idle, compute block 17 -- the sign word -- of each
+        // chunk, and after a barrier the evaluating lanes settle sign and
+        // acceptance.
         const bool split_sign = TM == 2 && !full && store_vals;
-        for (int h = 0; h < nch; h += SUB) {
+        // chunks per pass and words per chunk in ks
+        const int sub = (TM == 2 && (full || store_vals)) ? HALF : TILE_CHUNKS;
+        const int cw = TM != 2 ? CHUNK_WORDS : full ? CHUNK_WORDS : store_vals ? 72 : 64;
+        for (int h = 0; h < nch; h += sub) {
             __syncthreads();  // windows written / the previous pass's reads of ks done
             const long long c0 = chunk0 + h;
-            const int nsub = min(SUB, nch - h);
-            aes_chunks<TM, NT>(S, te, nsub, [c0](int k) { return c0 + k; }, ks, full,
+            const int nsub = min(sub, nch - h);
+            aes_chunks<TM, NT>(S, te, nsub, [c0](int k) { return c0 + k; }, ks, cw, full,
                                full ? 24 : (lazy_sign || split_sign) ? 16 : PART_BLOCKS, use_win ? win : nullptr, w0);
             __syncthreads();
             // ballot of the accept bits and the chunk's count and values
@@ -290,29 +301,28 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
                 }
             };
             if (split_sign) {
-                static_assert(TM != 2 || NT >= SUB * 9, "the sign blocks need SUB threads beside the SUB 8 lanes");
                 const int t = threadIdx.x, k = t >> 3, c = t & 7;
-                const bool eval = t < SUB * 8;
+                const bool eval = t < HALF * 8;
                 Cand cd{};
                 if (eval) {
-                    cd = eval_candidate(ks + k * CHUNK_WORDS, c, which, k < nsub, status, full, ulim,
+                    cd = eval_candidate(ks + k * cw, c, which, k < nsub, status, full, ulim,
                                         [](const uint32_t *) { return ~0u; });
-                } else if (t < SUB * 8 + nsub) {
-                    const int kb = t - SUB * 8;
+                } else if (t < HALF * 8 + nsub) {
+                    const int kb = t - HALF * 8;
                     uint32_t o[4];
                     aes_block<TM>(S, te, use_win ? win : nullptr, w0, (uint32_t)((c0 + kb) * 24 + 17), o);
-                    ks[kb * CHUNK_WORDS + 70] = o[2];
+                    ks[kb * cw + 70] = o[2];
                 }
                 __syncthreads();
                 if (eval) {
                     // with every sign 1, cd.acc was the Bernoulli bit alone
-                    cd.sgn = (ks[k * CHUNK_WORDS + 70] >> c) & 1u;
+                    cd.sgn = (ks[k * cw + 70] >> c) & 1u;
                     cd.acc = cd.acc && (cd.z != 0 || cd.sgn != 0);
                     commit(cd, k, c, k < nsub);
                 }
                 continue;
             }
-            for (int slot = threadIdx.x; slot < SUB * 8; slot += NT) {
+            for (int slot = threadIdx.x; slot < sub * 8; slot += NT) {
                 const int k = slot >> 3, c = slot & 7;
                 // with lazy_sign, block 17 of chunk k is computed by the asking
                 // lane alone (a zero candidate: about 1e-9 of S2/S3 candidates)
@@ -322,8 +332,7 @@ __global__ __launch_bounds__(NT) void k_gauss_candidates(const GaussStream *__re
                     aes_block<TM>(S, te, nullptr, 0, (uint32_t)((c0 + k) * 24 + 17), o);
                     return o[2];
                 };
-                commit(eval_candidate(ks + k * CHUNK_WORDS, c, which, k < nsub, status, full, ulim, sign_word,
-                                      lazy_sign),
+                commit(eval_candidate(ks + k * cw, c, which, k < nsub, status, full, ulim, sign_word, lazy_sign),
                        k, c, k < nsub);
             }
         }

@@ -63,6 +63,12 @@ struct GaussStream {
     long long first_base;    // compact streams: index of call 0 in the per-call first-chunk array
 };
 
+// Chunks per tile of the candidate pass (the host's tile_prefix counts tiles of
+// this size): 63, so that a tile's 16 AES blocks per chunk take two rounds of
+// 512 threads and its keystream (64 words per chunk) fits the LDS left beside
+// the 64 KiB T-tables of two workgroups.
+constexpr int GAUSS_TILE_CHUNKS = 63;
+
 // Candidate pass: for every chunk of every stream, AES + CDT + Bernoulli;
 // writes the accepted count and the accepted (z | sign << 63) values in order.
 // full: compute all 24 AES blocks of each chunk (else the 17 that candidates

@@ -118,7 +118,7 @@ size_t gauss_bytes(int which, int ncalls, int coef0_only)
 
 int GaussPlan::build(const StreamReq *reqs, int count, int lvl)
 {
-    constexpr int kTile = 64;
+    constexpr int kTile = GAUSS_TILE_CHUNKS;
     n = count;
     level = lvl;
     hs.assign(n, GaussStream{});

@@ -371,7 +371,8 @@ __global__ void k_decide(const uint32_t *__restrict__ kval, int R, int S, SvmMod
 // the order, so the 64 lanes form them in parallel; only the sum is ordered.
 // EXACT_INT (integer coefficients with sum |coef| * p < 2^53): every product
 // and partial sum is an exact integer, so any order gives the same bits and
-// the lanes reduce in a tree.  Otherwise the lanes stage the rounded products
+// the lanes reduce in a tree (with two classes, over 16-byte loads of the
+// row).  Otherwise the lanes stage the rounded products
 // of 512 terms at a time in LDS and lane 0 adds them in the reference order.
 // ---------------------------------------------------------------------------
 template <bool EXACT_INT>
@@ -384,6 +385,25 @@ __global__ __launch_bounds__(256) void k_decide_wave(const uint32_t *__restrict_
     const int r = blockIdx.x * 4 + wv;
     if (r >= R) return;  // whole wave; no block barriers below
     const uint32_t *kv = kval + (size_t)r * S;
+    if (EXACT_INT && m.nr_class == 2 && (S & 3) == 0) {
+        // Two classes: SV t (of either class) has coefficient sv_coef[t], so
+        // the decision value is one dot product over the row; each lane loads
+        // 4 consecutive key values (16 B) and their 4 coefficients (32 B).
+        const uint4 *kv4 = reinterpret_cast<const uint4 *>(kv);
+        const double2 *c2 = reinterpret_cast<const double2 *>(m.sv_coef);
+        double part = 0.0;
+#pragma unroll 4
+        for (int t = lane; t < (S >> 2); t += 64) {
+            const uint4 k = kv4[t];
+            const double2 a = c2[2 * t], b = c2[2 * t + 1];
+            part += (__dmul_rn(a.x, (double)k.x) + __dmul_rn(a.y, (double)k.y)) +
+                    (__dmul_rn(b.x, (double)k.z) + __dmul_rn(b.y, (double)k.w));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if (lane == 0) labels[r] = (double)m.label[__dsub_rn(part, m.rho[0]) > 0 ? 0 : 1];
+        return;
+    }
     int vote[SVM_MAX_CLASS];
     for (int i = 0; i < m.nr_class; ++i) vote[i] = 0;
     int p = 0;

@@ -308,9 +308,11 @@ void worker_main(cufe_server *s)
         if (progressed) continue;
         bool any_busy = s->slots[0].busy || s->slots[1].busy;
         if (!free_slot) {
-            // both slots run: wait for the older one
-            Slot &a = s->slots[0], &b = s->slots[1];
-            (void)hipEventSynchronize(a.busy ? a.done : b.done);
+            // both slots run: query both again (yielding the core), so that
+            // whichever batch ends first is retired and its slot refilled at
+            // once -- a blocking wait on one of them can sit behind a larger
+            // batch while the other slot is already free
+            std::this_thread::yield();
             continue;
         }
         std::unique_lock<std::mutex> lk(s->mu);

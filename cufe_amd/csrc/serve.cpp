@@ -272,9 +272,15 @@ void worker_main(cufe_server *s)
             {
                 std::lock_guard<std::mutex> lk(s->mu);
                 // admit queued requests in arrival order while the aligned
-                // layout fits the slot
+                // layout fits the slot; when the other slot is free too, this
+                // one takes the first half of the queue and the other slot the
+                // rest, so the two run side by side instead of one waiting
+                const bool other_free = !s->slots[0].busy && !s->slots[1].busy;
+                const size_t take =
+                    other_free && s->pending.size() > 1 ? (s->pending.size() + 1) / 2 : s->pending.size();
+                size_t taken = 0;
                 std::vector<std::pair<const Model *, int>> per_model;
-                while (!s->pending.empty()) {
+                while (!s->pending.empty() && taken < take) {
                     const ReqP &r = s->pending.front();
                     auto it = std::find_if(per_model.begin(), per_model.end(),
                                            [&](const std::pair<const Model *, int> &p) { return p.first == r->m; });
@@ -288,6 +294,7 @@ void worker_main(cufe_server *s)
                         it->second += r->n;
                     sl.reqs.push_back(r);
                     s->pending.pop_front();
+                    ++taken;
                     if (!s->batching) break;
                 }
                 if (!sl.reqs.empty()) sl.busy = true;

@@ -352,8 +352,8 @@ def main():
                     help="also replay the same requests this many times faster")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (rows would shard across GPUs, each holding "
+              "every functional key)", file=sys.stderr)
         sys.exit(2)
 
     import cufe_amd as cf

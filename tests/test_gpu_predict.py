@@ -123,6 +123,36 @@ def test_predict_row_subsets_and_chunking(gpu, keyed_ctx):
     assert np.array_equal(kv_all.astype(np.int64) % 50241, ip)
 
 
+def test_empty_and_ragged_ranges(gpu, keyed_ctx):
+    """Zero rows encrypt and predict as no-ops; one unaligned row, the last
+    rows of the store and the whole store agree; ranges outside the stored
+    rows are errors."""
+    from cufe_amd import CufeError
+
+    ctx, _, _ = keyed_ctx
+    rng = np.random.default_rng(51)
+    l, R, S = 16, 70, 9
+    model = make_model(gpu, rng, S, l)
+    ctx.rows_reserve(R, l)
+    ctx.encrypt_rows(np.zeros((0, l), np.uint32), h(b"g2"), h(b"g3"), nonce0=0)
+    assert ctx.rows_count() == 0
+    x = rng.integers(0, 5, (R, l)).astype(np.uint32)
+    ctx.encrypt_rows(x, h(b"g2"), h(b"g3"), nonce0=0)
+    assert ctx.rows_count() == R
+    ctx.set_model(model)
+    lab0, kv0, _ = ctx.predict(0, 0, want_kvalues=True)
+    assert lab0.shape == (0,) and kv0.shape == (0, S)
+    lab_all, kv_all, _ = ctx.predict(0, R, want_kvalues=True)
+    for r0, n in ((37, 1), (R - 1, 1), (64, R - 64), (31, 2)):
+        lab, kv, _ = ctx.predict(r0, n, want_kvalues=True)
+        assert np.array_equal(lab_all[r0:r0 + n], lab) and np.array_equal(kv_all[r0:r0 + n], kv), (r0, n)
+    for r0, n in ((0, R + 1), (R, 1), (-1, 1)):
+        with pytest.raises(CufeError):
+            ctx.predict(r0, n)
+    ip = (x.astype(np.int64) @ model.sv[:, :l].astype(np.int64).T) % 50241
+    assert np.array_equal(kv_all.astype(np.int64) % 50241, ip)
+
+
 def test_predict_a1a_rows(gpu, po, keyed_ctx):
     """Real data: the reference's a1a model (591 SVs) on 40 of its test rows;
     labels equal the plaintext SVM wherever the decryption is exact."""

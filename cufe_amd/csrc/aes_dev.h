@@ -63,6 +63,12 @@ __device__ __forceinline__ void aes_stage_tables_rep(uint32_t *lds_rep, const ui
 }
 
 __device__ __forceinline__ uint32_t ror32(uint32_t x, int n) { return __builtin_amdgcn_alignbit(x, x, n); }
+// a ^ b ^ c as one v_bitop3_b32 (truth table 0x96); the compiler does not
+// form it from two XORs by itself.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
 
 __device__ __forceinline__ void aes256_ctr_block_rep(const uint32_t *__restrict__ rk, const uint32_t *te_rep,
                                                      uint32_t nonce, uint32_t ctr, uint32_t (&o)[4])
@@ -147,10 +153,10 @@ template <class Tab>
 __device__ __forceinline__ void aes_round(const Tab &tb, uint32_t &s0, uint32_t &s1, uint32_t &s2, uint32_t &s3,
                                           const uint32_t *rk)
 {
-    const uint32_t t0 = tb.t0(s0, 3) ^ tb.t2(s2, 1) ^ ror32(tb.t0(s1, 2) ^ tb.t2(s3, 0), 8) ^ rk[0];
-    const uint32_t t1 = tb.t0(s1, 3) ^ tb.t2(s3, 1) ^ ror32(tb.t0(s2, 2) ^ tb.t2(s0, 0), 8) ^ rk[1];
-    const uint32_t t2 = tb.t0(s2, 3) ^ tb.t2(s0, 1) ^ ror32(tb.t0(s3, 2) ^ tb.t2(s1, 0), 8) ^ rk[2];
-    const uint32_t t3 = tb.t0(s3, 3) ^ tb.t2(s1, 1) ^ ror32(tb.t0(s0, 2) ^ tb.t2(s2, 0), 8) ^ rk[3];
+    const uint32_t t0 = xor3(tb.t0(s0, 3), tb.t2(s2, 1), rk[0]) ^ ror32(tb.t0(s1, 2) ^ tb.t2(s3, 0), 8);
+    const uint32_t t1 = xor3(tb.t0(s1, 3), tb.t2(s3, 1), rk[1]) ^ ror32(tb.t0(s2, 2) ^ tb.t2(s0, 0), 8);
+    const uint32_t t2 = xor3(tb.t0(s2, 3), tb.t2(s0, 1), rk[2]) ^ ror32(tb.t0(s3, 2) ^ tb.t2(s1, 0), 8);
+    const uint32_t t3 = xor3(tb.t0(s3, 3), tb.t2(s1, 1), rk[3]) ^ ror32(tb.t0(s0, 2) ^ tb.t2(s2, 0), 8);
     s0 = t0;
     s1 = t1;
     s2 = t2;

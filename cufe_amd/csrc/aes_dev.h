@@ -14,6 +14,13 @@ namespace ipfe {
 
 constexpr int AES_TABLE_WORDS = 4 * 256;
 
+// a ^ b ^ c as one v_bitop3_b32 (truth table 0x96); the compiler does not
+// form it from two XORs by itself.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ void aes_stage_tables(uint32_t *lds_te, const uint32_t *__restrict__ te_global)
 {
     for (int i = threadIdx.x; i < AES_TABLE_WORDS; i += blockDim.x) lds_te[i] = te_global[i];
@@ -30,10 +37,10 @@ __device__ __forceinline__ void aes256_ctr_block(const uint32_t *__restrict__ rk
     uint32_t s3 = ctr ^ rk[3];
 #pragma unroll
     for (int r = 1; r < 14; ++r) {
-        uint32_t t0 = T0[s0 >> 24] ^ T1[(s1 >> 16) & 255] ^ T2[(s2 >> 8) & 255] ^ T3[s3 & 255] ^ rk[4 * r];
-        uint32_t t1 = T0[s1 >> 24] ^ T1[(s2 >> 16) & 255] ^ T2[(s3 >> 8) & 255] ^ T3[s0 & 255] ^ rk[4 * r + 1];
-        uint32_t t2 = T0[s2 >> 24] ^ T1[(s3 >> 16) & 255] ^ T2[(s0 >> 8) & 255] ^ T3[s1 & 255] ^ rk[4 * r + 2];
-        uint32_t t3 = T0[s3 >> 24] ^ T1[(s0 >> 16) & 255] ^ T2[(s1 >> 8) & 255] ^ T3[s2 & 255] ^ rk[4 * r + 3];
+        uint32_t t0 = xor3(xor3(T0[s0 >> 24], T1[(s1 >> 16) & 255], T2[(s2 >> 8) & 255]), T3[s3 & 255], rk[4 * r]);
+        uint32_t t1 = xor3(xor3(T0[s1 >> 24], T1[(s2 >> 16) & 255], T2[(s3 >> 8) & 255]), T3[s0 & 255], rk[4 * r + 1]);
+        uint32_t t2 = xor3(xor3(T0[s2 >> 24], T1[(s3 >> 16) & 255], T2[(s0 >> 8) & 255]), T3[s1 & 255], rk[4 * r + 2]);
+        uint32_t t3 = xor3(xor3(T0[s3 >> 24], T1[(s0 >> 16) & 255], T2[(s1 >> 8) & 255]), T3[s2 & 255], rk[4 * r + 3]);
         s0 = t0; s1 = t1; s2 = t2; s3 = t3;
     }
     // Final round (SubBytes, ShiftRows, AddRoundKey): the S-box byte sits in
@@ -63,12 +70,6 @@ __device__ __forceinline__ void aes_stage_tables_rep(uint32_t *lds_rep, const ui
 }
 
 __device__ __forceinline__ uint32_t ror32(uint32_t x, int n) { return __builtin_amdgcn_alignbit(x, x, n); }
-// a ^ b ^ c as one v_bitop3_b32 (truth table 0x96); the compiler does not
-// form it from two XORs by itself.
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
-{
-    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
 
 __device__ __forceinline__ void aes256_ctr_block_rep(const uint32_t *__restrict__ rk, const uint32_t *te_rep,
                                                      uint32_t nonce, uint32_t ctr, uint32_t (&o)[4])
@@ -81,14 +82,14 @@ __device__ __forceinline__ void aes256_ctr_block_rep(const uint32_t *__restrict_
     uint32_t s3 = ctr ^ rk[3];
 #pragma unroll
     for (int r = 1; r < 14; ++r) {
-        uint32_t t0 = TL(s0 >> 24) ^ ror32(TL((s1 >> 16) & 255), 8) ^ ror32(TL((s2 >> 8) & 255), 16) ^
-                      ror32(TL(s3 & 255), 24) ^ rk[4 * r];
-        uint32_t t1 = TL(s1 >> 24) ^ ror32(TL((s2 >> 16) & 255), 8) ^ ror32(TL((s3 >> 8) & 255), 16) ^
-                      ror32(TL(s0 & 255), 24) ^ rk[4 * r + 1];
-        uint32_t t2 = TL(s2 >> 24) ^ ror32(TL((s3 >> 16) & 255), 8) ^ ror32(TL((s0 >> 8) & 255), 16) ^
-                      ror32(TL(s1 & 255), 24) ^ rk[4 * r + 2];
-        uint32_t t3 = TL(s3 >> 24) ^ ror32(TL((s0 >> 16) & 255), 8) ^ ror32(TL((s1 >> 8) & 255), 16) ^
-                      ror32(TL(s2 & 255), 24) ^ rk[4 * r + 3];
+        uint32_t t0 = xor3(xor3(TL(s0 >> 24), ror32(TL((s1 >> 16) & 255), 8), ror32(TL((s2 >> 8) & 255), 16)),
+                           ror32(TL(s3 & 255), 24), rk[4 * r]);
+        uint32_t t1 = xor3(xor3(TL(s1 >> 24), ror32(TL((s2 >> 16) & 255), 8), ror32(TL((s3 >> 8) & 255), 16)),
+                           ror32(TL(s0 & 255), 24), rk[4 * r + 1]);
+        uint32_t t2 = xor3(xor3(TL(s2 >> 24), ror32(TL((s3 >> 16) & 255), 8), ror32(TL((s0 >> 8) & 255), 16)),
+                           ror32(TL(s1 & 255), 24), rk[4 * r + 2]);
+        uint32_t t3 = xor3(xor3(TL(s3 >> 24), ror32(TL((s0 >> 16) & 255), 8), ror32(TL((s1 >> 8) & 255), 16)),
+                           ror32(TL(s2 & 255), 24), rk[4 * r + 3]);
         s0 = t0; s1 = t1; s2 = t2; s3 = t3;
     }
     // S-box byte of T0[x] is byte 1 (and byte 2)

@@ -40,3 +40,15 @@ def test_python_binding_rejects_missing_library(monkeypatch):
     monkeypatch.setattr(cufe_amd, "LIB_PATH", "/nonexistent/libcufe_ipfe.so")
     with pytest.raises(cufe_amd.CufeError):
         cufe_amd.lib()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    so, before it touches a device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""

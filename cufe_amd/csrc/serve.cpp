@@ -173,11 +173,22 @@ int launch_batch(cufe_server *s, Slot &sl)
         HIP_OK(hipMemcpyAsync(d_c0 + c0_words, src, c0_bytes, hipMemcpyHostToDevice, sl.st));
         c0_words += (size_t)r.n * r.l * 3;
     }
-    for (size_t i = 0; i < sl.reqs.size(); ++i) {
-        const Req &r = *sl.reqs[i];
-        if (sl.rows.put(sl.req_row[i], r.n, sl.d_stage.get() + (size_t)sl.req_row[i] * kPoly, d_c0 + c0_off[i], sl.st,
-                        r.l))
+    // one conversion launch per run of requests whose rows and c0 words follow
+    // each other and whose rows have the same length (every request of a
+    // one-model batch)
+    for (size_t i = 0; i < sl.reqs.size();) {
+        const int l = sl.reqs[i]->l;
+        int n = sl.reqs[i]->n;
+        size_t j = i + 1;
+        while (j < sl.reqs.size() && sl.reqs[j]->l == l && sl.req_row[j] == sl.req_row[i] + n &&
+               c0_off[j] == c0_off[i] + (size_t)n * l * 3) {
+            n += sl.reqs[j]->n;
+            ++j;
+        }
+        if (sl.rows.put(sl.req_row[i], n, sl.d_stage.get() + (size_t)sl.req_row[i] * kPoly, d_c0 + c0_off[i], sl.st,
+                        l))
             return -1;
+        i = j;
     }
     // small groups take the low-latency split-K launch unless the other slot
     // runs a large batch (under load the split launch would slow it more than

@@ -13,8 +13,9 @@ Ciphertexts and keys are in HBM before the timed region starts.  Encryption of
 the rows and functional-key generation are timed separately and reported.
 
 Extra fields: the plain per-call GPU form (the four reference C-ABI calls per
-row, svm_predict_gpu), the roofline of the dominant kernel (the fp64
-contraction, HIP events), and the CPU oracle on a bounded sample.
+row, svm_predict_gpu), the roofline of the dominant kernel (the int8
+matrix-core contraction, timed with HIP events on its stream; the fp64
+contraction when keys need it), and the CPU oracle on a bounded sample.
 """
 from __future__ import annotations
 
@@ -219,8 +220,15 @@ def serving(cf, ctx, model, x, args, rng):
             srv.add_model(m)
         srv.start()
         log(f"serving replay ({mode}): {n} requests, {total_rows} rows at {args.serve_rate * speed:.0f} req/s")
+        # untimed warm-up: the first 200 requests once, so that one-time costs
+        # of the process (first launches, first use of host buffers) stay out
+        # of every mode's latencies alike
+        w = min(200, n)
+        srv.replay(arrival[:w] / speed, mids[:w], rows[:w])
+        st0 = srv.stats()
         labels, lat, makespan, failed = srv.replay(arrival / speed, mids, rows)
         st = srv.stats()
+        st["batches"] -= st0["batches"]
         srv.close()
         bad = sum(int(not np.array_equal(labels[i], refs[mids[i]][spans[i][0]:spans[i][0] + spans[i][1]]))
                   for i in range(n))
@@ -482,15 +490,18 @@ def main():
                    "predictions_per_s_with_encryption": R / (t_enc + ms_per_step / 1e3)},
         "decryption_check": {"pairs": chk * S, "fraction_equal_mod_p": dec_ok, "zero_decrypted_as_p": dec_p},
     }
+    # serving right after the prediction leg: run after the single-operation
+    # and batched-encryption legs (about 82 GB of ciphertexts allocated and
+    # released) the first replay's p99 was 0.39-0.46 ms instead of 0.28 ms
+    if args.serve_requests > 0:
+        out["serving"] = serving(cf, ctx, model, x, args, rng)
+        out["p50_ms"] = out["serving"]["batched"]["p50_ms"]
+        out["p99_ms"] = out["serving"]["batched"]["p99_ms"]
     if not args.no_single_ops:
         log("single-operation kernels")
         out["single_ops"] = single_ops(cf, ctx, rng)
     if args.enc_messages > 0:
         out["batched_encrypt"] = batched_encrypt(cf, ctx, args, rng)
-    if args.serve_requests > 0:
-        out["serving"] = serving(cf, ctx, model, x, args, rng)
-        out["p50_ms"] = out["serving"]["batched"]["p50_ms"]
-        out["p99_ms"] = out["serving"]["batched"]["p99_ms"]
     if args.plain_rows > 0:
         log("plain per-call GPU form")
         out["plain_gpu"] = plain_gpu(cf, ctx, model, x, args.plain_rows)

@@ -124,8 +124,13 @@ def cpu_baseline_all_cores(cf, ctx, model, x_row, seconds: float):
         np.savez(path, c=c, msk=msk, sv=model.sv)
         k = 2
         jobs = [(path, (i * k) % (model.l - k), k, seconds) for i in range(cores)]
+        # close + join: the workers exit on their own once the map is done.
+        # Leaving the `with` block first would call Pool.terminate(), which
+        # signals every worker (under a profiler that signal aborts them).
         with mp.get_context("spawn").Pool(cores) as pool:
             res = pool.map(_cpu_worker, jobs)
+            pool.close()
+            pool.join()
     pairs = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
     rate_pairs = pairs / wall
@@ -151,96 +156,115 @@ def plain_gpu(cf, ctx, model, x, nrows: int):
             "ms_per_prediction": 1e3 * float(np.mean(times))}
 
 
-def serving(cf, ctx, model, x, args, rng):
-    """Open-loop Poisson replay through the continuous-batching server, then the
-    same arrivals with one request per batch.  Four models (100, 600 and 2000
-    SVs over 123-dim rows; 1000 SVs over rows of a dimension drawn from
-    U{16..785}); request sizes U{1..64} rows; model uniform per request.
-    Every served label is checked against cufe_ctx_predict on the same rows."""
-    mpk, msk = ctx.get_keys()
-    dim2 = int(rng.integers(16, 786))
-    pool1 = min(1024, x.shape[0])
-    cl1, c01 = ctx.export_rows(0, pool1)
-    ctx2 = cf.Context()
-    ctx2.load_keys(mpk, msk)
-    pool2 = 512
-    ctx2.rows_reserve(pool2, dim2)
-    x2 = np.zeros((pool2, dim2), np.uint32)
-    for r in range(pool2):
-        x2[r, rng.choice(dim2, min(14, dim2), replace=False)] = 1
-    ctx2.encrypt_rows(x2, h(b"serve-2"), h(b"serve-3"), nonce0=0)
-    cl2, c02 = ctx2.export_rows(0, pool2)
+class ServingBench:
+    """Open-loop Poisson replays through the continuous-batching server.  Four
+    models (100, 600 and 2000 SVs over 123-dim rows; 1000 SVs over rows of a
+    dimension drawn from U{16..785}); request sizes U{1..64} rows; model
+    uniform per request.  Every served label is checked against
+    cufe_ctx_predict on the same rows."""
 
-    def pinned(a):
-        # request rows live in pinned host memory, so the server DMAs them
-        # straight from the caller's buffers
-        p = cf.host_empty(a.shape, a.dtype)
-        p[...] = a
-        return p
+    def __init__(self, cf, ctx, model, x, args, rng):
+        self.cf, self.ctx, self.args = cf, ctx, args
+        mpk, msk = ctx.get_keys()
+        dim2 = int(rng.integers(16, 786))
+        pool1 = min(1024, x.shape[0])
+        cl1, c01 = ctx.export_rows(0, pool1)
+        self.ctx2 = ctx2 = cf.Context()
+        ctx2.load_keys(mpk, msk)
+        pool2 = 512
+        ctx2.rows_reserve(pool2, dim2)
+        x2 = np.zeros((pool2, dim2), np.uint32)
+        for r in range(pool2):
+            x2[r, rng.choice(dim2, min(14, dim2), replace=False)] = 1
+        ctx2.encrypt_rows(x2, h(b"serve-2"), h(b"serve-3"), nonce0=0)
+        cl2, c02 = ctx2.export_rows(0, pool2)
 
-    cl1, c01, cl2, c02 = pinned(cl1), pinned(c01), pinned(cl2), pinned(c02)
+        def pinned(a):
+            # request rows live in pinned host memory, so the server DMAs them
+            # straight from the caller's buffers
+            p = cf.host_empty(a.shape, a.dtype)
+            p[...] = a
+            return p
 
-    def svm(S, dim, width):
-        svx = np.zeros((S, width), np.uint32)
-        for v in range(S):
-            svx[v, rng.choice(dim, min(14, dim), replace=False)] = 1
-        co = rng.choice(np.concatenate([np.arange(-16, 0), np.arange(1, 17)]), S).astype(np.float64)[None, :]
-        return cf.SvmModel(2, [S // 2, S - S // 2], [1, -1], co, [0.5], svx)
+        cl1, c01, cl2, c02 = pinned(cl1), pinned(c01), pinned(cl2), pinned(c02)
 
-    models = [(svm(100, 123, 124), 1), (svm(600, 123, 124), 1), (model, 1), (svm(1000, dim2, dim2), 2)]
-    pools = {1: (ctx, cl1, c01, pool1), 2: (ctx2, cl2, c02, pool2)}
-    refs = []
-    for m, p in models:
-        c = pools[p][0]
-        c.set_model(m)
-        refs.append(c.predict(0, pools[p][3])[0])
-    ctx.set_model(model)
+        def svm(S, dim, width):
+            svx = np.zeros((S, width), np.uint32)
+            for v in range(S):
+                svx[v, rng.choice(dim, min(14, dim), replace=False)] = 1
+            co = rng.choice(np.concatenate([np.arange(-16, 0), np.arange(1, 17)]), S).astype(np.float64)[None, :]
+            return cf.SvmModel(2, [S // 2, S - S // 2], [1, -1], co, [0.5], svx)
 
-    n = args.serve_requests
-    arrival = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
-    mids, rows, spans = [], [], []
-    for _ in range(n):
-        mid = int(rng.integers(0, len(models)))
-        _, cl, c0, pr = pools[models[mid][1]]
-        k = int(rng.integers(1, 65))
-        r0 = int(rng.integers(0, pr - k + 1))
-        mids.append(mid)
-        rows.append((cl[r0:r0 + k], c0[r0:r0 + k]))
-        spans.append((r0, k))
-    total_rows = int(sum(k for _, k in spans))
-    res = {"arrival_rate_req_s": args.serve_rate, "requests": n, "rows": total_rows,
-           "models": [{"svs": m.l, "row_slots": int(pools[p][2].shape[1])} for m, p in models]}
-    runs = [("batched", True, 1), ("one_request_per_batch", False, 1)]
-    if args.serve_burst > 1:
-        runs += [(f"batched_x{args.serve_burst}", True, args.serve_burst),
-                 (f"one_request_per_batch_x{args.serve_burst}", False, args.serve_burst)]
-    for mode, batching, speed in runs:
-        srv = cf.Server(ctx, max_batch_rows=4096, batching=batching)
-        for m, _ in models:
+        self.models = [(svm(100, 123, 124), 1), (svm(600, 123, 124), 1), (model, 1), (svm(1000, dim2, dim2), 2)]
+        pools = {1: (ctx, cl1, c01, pool1), 2: (ctx2, cl2, c02, pool2)}
+        self.refs = []
+        for m, p in self.models:
+            c = pools[p][0]
+            c.set_model(m)
+            self.refs.append(c.predict(0, pools[p][3])[0])
+        ctx.set_model(model)
+
+        n = args.serve_requests
+        self.arrival = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+        self.mids, self.rows, self.spans = [], [], []
+        for _ in range(n):
+            mid = int(rng.integers(0, len(self.models)))
+            _, cl, c0, pr = pools[self.models[mid][1]]
+            k = int(rng.integers(1, 65))
+            r0 = int(rng.integers(0, pr - k + 1))
+            self.mids.append(mid)
+            self.rows.append((cl[r0:r0 + k], c0[r0:r0 + k]))
+            self.spans.append((r0, k))
+        self.total_rows = int(sum(k for _, k in self.spans))
+        self.info = {"arrival_rate_req_s": args.serve_rate, "requests": n, "rows": self.total_rows,
+                     "models": [{"svs": m.l, "row_slots": int(pools[p][2].shape[1])} for m, p in self.models]}
+
+    def run(self, mode: str, batching: bool, speed: float) -> dict:
+        cf, n, args = self.cf, len(self.arrival), self.args
+        arrival, mids, rows = self.arrival, self.mids, self.rows
+        srv = cf.Server(self.ctx, max_batch_rows=4096, batching=batching)
+        for m, _ in self.models:
             srv.add_model(m)
         srv.start()
-        log(f"serving replay ({mode}): {n} requests, {total_rows} rows at {args.serve_rate * speed:.0f} req/s")
+        log(f"serving replay ({mode}): {n} requests, {self.total_rows} rows at {args.serve_rate * speed:.0f} req/s")
         # untimed warm-up: the first 200 requests once, so that one-time costs
         # of the process (first launches, first use of host buffers) stay out
         # of every mode's latencies alike
         w = min(200, n)
-        srv.replay(arrival[:w] / speed, mids[:w], rows[:w])
-        st0 = srv.stats()
+        _, _, _, wfailed = srv.replay(arrival[:w] / speed, mids[:w], rows[:w])
+        if wfailed:
+            raise RuntimeError(f"serving warm-up ({mode}): {wfailed} of {w} requests failed")
+        srv.reset_stats()  # every stat below covers the timed replay only
         labels, lat, makespan, failed = srv.replay(arrival / speed, mids, rows)
         st = srv.stats()
-        st["batches"] -= st0["batches"]
         srv.close()
-        bad = sum(int(not np.array_equal(labels[i], refs[mids[i]][spans[i][0]:spans[i][0] + spans[i][1]]))
+        bad = sum(int(not np.array_equal(labels[i], self.refs[mids[i]][self.spans[i][0]:self.spans[i][0] +
+                                                                         self.spans[i][1]]))
                   for i in range(n))
-        res[mode] = {"arrival_rate_req_s": args.serve_rate * speed, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                     "p999_ms": float(np.percentile(lat, 99.9)), "mean_ms": float(np.mean(lat)),
-                     "throughput_req_s": n / makespan, "throughput_rows_s": total_rows / makespan,
-                     "makespan_s": makespan, "batches": st["batches"], "max_batch_rows": st["max_batch_rows"],
-                     "slot_cap_rows": st["cap_rows"], "failed": failed,
-                     "requests_differing_from_ctx_predict": bad}
-        log(f"  p50 {res[mode]['p50_ms']:.2f} ms p99 {res[mode]['p99_ms']:.2f} ms "
-            f"{res[mode]['throughput_req_s']:.0f} req/s, {st['batches']} batches, mismatches {bad}")
-    ctx2.close()
+        r = {"arrival_rate_req_s": args.serve_rate * speed, "p50_ms": float(np.percentile(lat, 50)),
+             "p99_ms": float(np.percentile(lat, 99)), "p999_ms": float(np.percentile(lat, 99.9)),
+             "mean_ms": float(np.mean(lat)), "throughput_req_s": n / makespan,
+             "throughput_rows_s": self.total_rows / makespan, "makespan_s": makespan, "batches": st["batches"],
+             "max_batch_rows": st["max_batch_rows"], "slot_cap_rows": st["cap_rows"], "failed": failed,
+             "requests_differing_from_ctx_predict": bad}
+        log(f"  p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms {r['throughput_req_s']:.0f} req/s, "
+            f"{st['batches']} batches, mismatches {bad}")
+        return r
+
+    def close(self):
+        self.ctx2.close()
+
+
+def serving(sb: ServingBench, args) -> dict:
+    """Batched replay at the offered rate, the same arrivals with one request
+    per batch, and both again at serve_burst times the rate."""
+    res = dict(sb.info)
+    runs = [("batched", True, 1), ("one_request_per_batch", False, 1)]
+    if args.serve_burst > 1:
+        runs += [(f"batched_x{args.serve_burst:g}", True, args.serve_burst),
+                 (f"one_request_per_batch_x{args.serve_burst:g}", False, args.serve_burst)]
+    for mode, batching, speed in runs:
+        res[mode] = sb.run(mode, batching, speed)
     return res
 
 
@@ -358,6 +382,8 @@ def main():
                     help="run a 1 GiB device copy first (known bytes, to scale HBM counters)")
     ap.add_argument("--serve-burst", type=float, default=10.0,
                     help="also replay the same requests this many times faster")
+    ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
+                    help="file that receives the full results (the stdout line is a summary)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (rows would shard across GPUs, each holding "
@@ -481,7 +507,6 @@ def main():
                                f"N=4096, 3 RNS limbs, key length 785",
                    "rows": R, "svs": S, "row_slots": l, "parallelism": "single GPU"},
         "roofline": roof,
-        "autotune": cf.autotune_report().strip().split("\n"),
         "phases": {"setup_s": t_setup, "encrypt_s": t_enc, "encrypt_rows_per_s": R / t_enc,
                    "keygen_ms": t_keygen * 1e3,
                    "contract_ms_per_limb": [round(t / args.steps, 4) for t in limb_ms],
@@ -490,11 +515,13 @@ def main():
                    "predictions_per_s_with_encryption": R / (t_enc + ms_per_step / 1e3)},
         "decryption_check": {"pairs": chk * S, "fraction_equal_mod_p": dec_ok, "zero_decrypted_as_p": dec_p},
     }
-    # serving right after the prediction leg: run after the single-operation
-    # and batched-encryption legs (about 82 GB of ciphertexts allocated and
-    # released) the first replay's p99 was 0.39-0.46 ms instead of 0.28 ms
+    # serving right after the prediction leg, and the batched replay once more
+    # after the single-operation, batched-encryption and plain legs (about 82 GB
+    # of ciphertexts allocated and released in between): both p99s are reported
+    sb = None
     if args.serve_requests > 0:
-        out["serving"] = serving(cf, ctx, model, x, args, rng)
+        sb = ServingBench(cf, ctx, model, x, args, rng)
+        out["serving"] = serving(sb, args)
         out["p50_ms"] = out["serving"]["batched"]["p50_ms"]
         out["p99_ms"] = out["serving"]["batched"]["p99_ms"]
     if not args.no_single_ops:
@@ -506,6 +533,9 @@ def main():
         log("plain per-call GPU form")
         out["plain_gpu"] = plain_gpu(cf, ctx, model, x, args.plain_rows)
         out["batched_vs_plain"] = value / out["plain_gpu"]["value"]
+    if sb is not None:
+        out["serving"]["batched_after_other_legs"] = sb.run("batched_after_other_legs", True, 1)
+        sb.close()
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 core)")
         out["cpu_baseline"] = cpu_baseline(cf, ctx, model, x[0], args.cpu_seconds)
@@ -513,7 +543,82 @@ def main():
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(cf, ctx, model, x[0], args.cpu_seconds / 2)
     ctx.close()
     out["autotune"] = cf.autotune_report().strip().split("\n")  # every shape tuned, serving batches included
-    print(json.dumps(out))
+    detail = write_detail(out, args.detail)
+    print(compact_line(out, detail), flush=True)
+
+
+def write_detail(out: dict, path: str) -> str:
+    """The full result (autotuner report, every leg) as JSON in a file; the
+    stdout line carries only the summary."""
+    path = path if os.path.isabs(path) else os.path.join(ROOT, path)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    log(f"full results in {path}")
+    return os.path.relpath(path, ROOT)
+
+
+LINE_LIMIT = 4000  # bytes of the final stdout line
+
+
+def _r(v, nd=4):
+    return float(f"{v:.{nd}g}") if isinstance(v, float) else v
+
+
+def compact_line(out: dict, detail: str | None = None) -> str:
+    """The one JSON line the bench prints last: the headline metric, config,
+    roofline of the dominant kernel, CPU baseline, the plain per-call form,
+    serving p50/p99 and a few per-leg figures -- under LINE_LIMIT bytes."""
+    keep = ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
+            "vs_baseline", "dtype", "data", "config", "p50_ms", "p99_ms", "batched_vs_plain")
+    line = {k: _r(out[k]) for k in keep if k in out}
+    line["value"] = out["value"]
+    line["ms_per_step"] = out["ms_per_step"]
+    roof = out.get("roofline")
+    if roof:
+        rk = ("bound", "kernel", "variant", "achieved", "peak", "unit", "frac", "traffic", "avg_launch_ms",
+              "work_per_launch")
+        line["roofline"] = {k: _r(roof[k]) for k in rk if k in roof}
+        if "limb0" in roof:
+            line["roofline"]["limb0"] = {k: _r(roof["limb0"][k]) for k in ("variant", "achieved", "frac",
+                                                                          "avg_launch_ms") if k in roof["limb0"]}
+    for key in ("cpu_baseline", "cpu_baseline_all_cores"):
+        if key in out:
+            line[key] = {k: _r(v) for k, v in out[key].items()}
+    if "plain_gpu" in out:
+        line["plain_gpu"] = {k: _r(out["plain_gpu"][k]) for k in ("value", "unit", "ms_per_prediction")}
+    sv = out.get("serving")
+    if sv:
+        line["serving"] = {m: {k: _r(sv[m][k]) for k in ("p50_ms", "p99_ms", "throughput_req_s", "batches",
+                                                          "failed", "requests_differing_from_ctx_predict")}
+                           for m in sv if isinstance(sv[m], dict) and "p99_ms" in sv[m]}
+    ph = out.get("phases")
+    if ph:
+        line["phases"] = {k: _r(ph[k]) for k in ("encrypt_rows_per_s", "keygen_ms", "contract_ms_per_limb",
+                                                 "predictions_per_s_with_encryption") if k in ph}
+    if "decryption_check" in out:
+        line["decryption_check"] = out["decryption_check"]
+    be = out.get("batched_encrypt")
+    if be:
+        line["batched_encrypt"] = {k: _r(be[k]) for k in ("encryptions_per_s_graph", "eager_ms", "graph_ms",
+                                                          "batched_vs_plain", "graph_equals_eager",
+                                                          "message0_equals_plain_abi_prefix") if k in be}
+    so = out.get("single_ops")
+    if so:
+        line["single_ops"] = {k: _r(v["limb_transforms_per_s"] if isinstance(v, dict) and
+                                    "limb_transforms_per_s" in v else v.get("polys_per_s"))
+                              for k, v in so.items() if isinstance(v, dict) and
+                              ("limb_transforms_per_s" in v or "polys_per_s" in v)}
+    if detail:
+        line["detail"] = detail
+    text = json.dumps(line, separators=(",", ":"))
+    # shed the optional blocks, least important first, until the line fits
+    for key in ("single_ops", "batched_encrypt", "decryption_check", "phases", "serving", "cpu_baseline_all_cores"):
+        if len(text.encode()) <= LINE_LIMIT:
+            break
+        line.pop(key, None)
+        text = json.dumps(line, separators=(",", ":"))
+    return text
 
 
 if __name__ == "__main__":

@@ -23,7 +23,9 @@ Q = (16760833, 2147352577, 2130706433)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcufe_ipfe.so")
+BENCH_LIB_PATH = os.path.join(_HERE, "libcufe_bench.so")  # measurement library (microbenchmarks)
 _lib = None
+_bench_lib = None
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
@@ -62,7 +64,6 @@ def lib() -> C.CDLL:
         "cufe_time_ntt": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "cufe_time_gauss": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
         "cufe_time_gauss_calls": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
-        "cufe_microbench": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
         "cufe_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
         "cufe_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
         "cufe_contract_variants": (C.c_int, [C.c_int, C.POINTER(C.c_ulonglong)]),
@@ -224,8 +225,22 @@ def microbench(kind: int) -> float:
     7: NTT butterflies in registers G/s, 8: int8 MFMA T ops/s on random operands,
     sustained over 8 back-to-back launches, 9: AES (T0 and T2 replicated) G blocks/s."""
     r = C.c_double()
-    _check(lib().cufe_microbench(kind, C.byref(r)), "microbench")
+    _check(bench_lib().cufe_microbench(kind, C.byref(r)), "microbench")
     return r.value
+
+
+def bench_lib() -> C.CDLL:
+    """The measurement library beside the product library (raises if missing)."""
+    global _bench_lib
+    if _bench_lib is None:
+        lib()
+        if not os.path.exists(BENCH_LIB_PATH):
+            raise CufeError(f"{BENCH_LIB_PATH} is missing: build it with `make -C cufe_amd/csrc`")
+        lb = C.CDLL(BENCH_LIB_PATH)
+        lb.cufe_microbench.restype = C.c_int
+        lb.cufe_microbench.argtypes = [C.c_int, C.POINTER(C.c_double)]
+        _bench_lib = lb
+    return _bench_lib
 
 
 class _PinnedBlock:
@@ -584,6 +599,7 @@ def _server_sigs(lb):
         "cufe_server_replay": (C.c_int, [p, C.c_int, _f64p, _i32p, _i32p, _i32p, pp, pp, pp, _f64p,
                                          C.POINTER(C.c_double)]),
         "cufe_server_stats": (C.c_int, [p, C.POINTER(C.c_longlong)]),
+        "cufe_server_reset_stats": (C.c_int, [p]),
         "cufe_server_destroy": (None, [p]),
     }
     for name, (res, args) in sig.items():
@@ -672,6 +688,10 @@ class Server:
         _check(lib().cufe_server_stats(self._h, out), "server_stats")
         keys = ("requests", "rows", "batches", "failed", "max_batch_rows", "cap_rows", "model_bytes", "slot_bytes")
         return dict(zip(keys, list(out)))
+
+    def reset_stats(self):
+        """Zero the request, row, batch, failure and largest-batch counters."""
+        _check(lib().cufe_server_reset_stats(self._h), "server_reset_stats")
 
     def close(self):
         if self._h:

@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
 #include <vector>
 
 #include "../../include/cufe_ipfe.h"
@@ -283,10 +284,22 @@ extern "C" double cufe_svm_predict_gpu(const cufe_svm_model *model,
                                        uint32_t c[CUFE_SIFE_L + 1][CUFE_SIFE_NMODULI][CUFE_SIFE_N],
                                        const uint32_t msk[CUFE_SIFE_L][CUFE_SIFE_NMODULI][CUFE_SIFE_N], double *kvalue)
 {
+    if (!model || !c || !msk || model->l < 1 || model->nr_class < 2) {
+        ipfe_set_error("cufe_svm_predict_gpu: bad arguments");
+        return std::nan("");
+    }
     const int l = model->l;
     std::vector<uint32_t> sk((size_t)l * kPoly), dy((size_t)l * kPoly);
-    rlwe_sife_keygen_gpu(model->sv, msk, sk.data(), l, nullptr);
-    rlwe_sife_decrypt_gmp_gpu(c, model->sv, sk.data(), dy.data(), l, nullptr);
+    // the steps of rlwe_sife_keygen_gpu and rlwe_sife_decrypt_gmp_gpu, with
+    // their status checked: a failed device call yields NaN, never a label
+    // voted from unwritten key values
+    if (do_keygen(model->sv, &msk[0][0][0], sk.data(), l, IPFE_L, nullptr) ||
+        do_decrypt(&c[0][0][0], model->sv, sk.data(), dy.data(), l, IPFE_L, nullptr)) {
+        report("cufe_svm_predict_gpu");
+        if (kvalue)
+            for (int i = 0; i < l; ++i) kvalue[i] = std::nan("");
+        return std::nan("");
+    }
     std::vector<double> kv(l);
     for (int i = 0; i < l; ++i) {
         // round_extract_gmp2 reconstructs every coefficient and keeps only the first

@@ -5,6 +5,7 @@
 
 #include <vector>
 
+#include "../../include/cufe_bench.h"
 #include "../../include/cufe_ipfe.h"
 #include "aes_dev.h"
 #include "ntt_core.h"

@@ -604,6 +604,19 @@ extern "C" int cufe_server_stats(cufe_server *s, long long *out8)
     return 0;
 }
 
+// Zeroes the counters cufe_server_stats reports (requests, rows, batches,
+// failed requests, largest batch), e.g. after an untimed warm-up.
+extern "C" int cufe_server_reset_stats(cufe_server *s)
+{
+    if (!s) {
+        ipfe_set_error("cufe_server_reset_stats: null server");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st_requests = s->st_rows = s->st_batches = s->st_failed = s->st_batch_rows_max = 0;
+    return 0;
+}
+
 // Completes every queued request, stops the worker and frees the server.
 extern "C" void cufe_server_destroy(cufe_server *s)
 {

@@ -90,7 +90,9 @@ typedef struct {
 /* svm_predict_gpu = svm_predict5 (main.c:270-390) for one ciphertext
  * c[786][3][4096]: keygen for every SV, decryption, round_extract, vote.
  * Returns the label; kvalue (optional, [l]) receives the decrypted inner
- * products.  The plain per-call form: keys are recomputed on every call. */
+ * products.  The plain per-call form: keys are recomputed on every call.
+ * On failure it returns NaN (and fills kvalue with NaN) with the reason in
+ * cufe_last_error(). */
 double cufe_svm_predict_gpu(const cufe_svm_model *model,
                             uint32_t c[CUFE_SIFE_L + 1][CUFE_SIFE_NMODULI][CUFE_SIFE_N],
                             const uint32_t msk[CUFE_SIFE_L][CUFE_SIFE_NMODULI][CUFE_SIFE_N], double *kvalue);
@@ -123,14 +125,6 @@ int cufe_aes256ctr(const unsigned char key[32], uint16_t nonce, uint32_t ctr0, u
 /* CRT + rounding of coefficient coef of nvec RNS polys d[nvec][3][4096]. */
 int cufe_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r);
 
-/* Measured ceilings of this GPU: kind 0 = fp64 FMA (TFLOP/s), 1 = 32-bit
- * mul_lo + mul_hi + add (T int-ops/s), 2 = device copy (GB/s read + write),
- * 3 = AES-256-CTR keystream (G blocks/s), 4 = the same with the
- * bank-replicated table, 5 = int8 MFMA 32x32x32 (T ops/s), 6 = fp64 MFMA
- * 16x16x4 (TFLOP/s), 7 = NTT butterflies in registers (G/s), 8 = int8 MFMA
- * on random operands, sustained over 8 back-to-back launches (T ops/s), 9 =
- * AES-256-CTR with T0 and T2 replicated per bank (G blocks/s). */
-int cufe_microbench(int kind, double *result);
 
 /* The scheme at a vector length l <= 785 (uneven-length batching).  With
  * zero-padding, a length-l encryption equals the first l polynomials plus c_L
@@ -276,6 +270,9 @@ int cufe_server_replay(cufe_server *srv, int nreq, const double *arrival_s, cons
 /* requests, rows, batches, failed, largest batch rows, slot capacity rows,
  * model key bytes, bytes per slot. */
 int cufe_server_stats(cufe_server *srv, long long *out8);
+/* Zeroes the request, row, batch, failure and largest-batch counters (after
+ * an untimed warm-up, so that reported stats cover the timed requests only). */
+int cufe_server_reset_stats(cufe_server *srv);
 /* Completes queued requests, stops the worker, frees device memory. */
 void cufe_server_destroy(cufe_server *srv);
 /* Pinned host memory (hipHostMalloc).  Request rows that lie inside such a

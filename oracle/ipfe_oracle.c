@@ -634,6 +634,53 @@ double orc_round_extract_coef0(const uint32_t *d)
     return (double)orc_round_extract(lo, hi);
 }
 
+/* Key values of many (row, key) pairs from coefficient 0 alone: what
+ * svm_predict_cpu (main.c:170-207) keeps of each decryption, kvalue =
+ * round_extract(CRT(d_y[.][0])).  Per limb, d_y[0] = sub_mod(acc, prod0)
+ * (src/rlwe_sife.c:265-277) with
+ *   acc   = the fold acc <- mod_prime(y_i c_i[0] + acc) over i < ly, where slots
+ *           at and beyond the row length l hold the encryption of 0, whose
+ *           product with y_i = 0 is 0 (the key must be 0 there);
+ *   prod0 = coefficient 0 of the negacyclic product c_L * sk_y, i.e.
+ *           a_0 b_0 - sum_{j>=1} a_j b_{N-j} mod q (the closed form of
+ *           GS_reverse(CT_forward(a) . CT_forward(b))[0]), canonical.
+ * The reference's INTT may return q instead of 0 for prod0; that changes d_y
+ * only when acc = q (a fold ending on a non-zero slot).  Such pairs are
+ * counted in the return value and their key value is set to 0xffffffff.
+ * Layouts: cl [R][3][N], c0 [R][l][3], y [S][ly], sk [S][3][N], kv [R][S]. */
+int orc_kvalues_coef0(const uint32_t *cl, const uint32_t *c0, int R, int l, const uint32_t *y, int ly,
+                      const uint32_t *sk, int S, uint32_t *kv)
+{
+    int ambiguous = 0;
+    for (int r = 0; r < R; ++r)
+        for (int v = 0; v < S; ++v) {
+            const uint32_t *yv = y + (size_t)v * ly;
+            uint32_t d[3];
+            int amb = 0;
+            for (int j = 0; j < 3; ++j) {
+                const uint64_t q = Q[j];
+                uint32_t acc = 0;
+                for (int i = 0; i < ly; ++i) {
+                    const uint64_t ci = i < l ? c0[((size_t)r * l + i) * 3 + j] : 0;
+                    acc = orc_mod_prime((uint64_t)orc_mod_prime(yv[i], j) * ci + acc, j);
+                }
+                const uint32_t *a = cl + ((size_t)r * 3 + j) * ORC_N;
+                const uint32_t *b = sk + ((size_t)v * 3 + j) * ORC_N;
+                unsigned __int128 minus = 0;
+                for (int k = 1; k < ORC_N; ++k) minus += (uint64_t)a[k] * b[ORC_N - k];
+                const uint64_t plus = (uint64_t)(((unsigned __int128)a[0] * b[0]) % q);
+                const uint64_t prod0 = (plus + q - (uint64_t)(minus % q)) % q;
+                if (acc == q) amb = 1;
+                d[j] = orc_sub_mod(acc, (uint32_t)prod0, j);
+            }
+            uint64_t lo, hi;
+            orc_crt_reverse(d[0], d[1], d[2], &lo, &hi);
+            kv[(size_t)r * S + v] = amb ? 0xffffffffu : orc_round_extract(lo, hi);
+            ambiguous += amb;
+        }
+    return ambiguous;
+}
+
 /* One-vs-one vote (main.c:338-389). */
 double orc_svm_decide(int nr_class, int l_sv, const int *nSV, const int *label,
                       const double *sv_coef, const double *rho, const double *kvalue, int use_fma)

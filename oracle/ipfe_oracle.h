@@ -92,6 +92,15 @@ uint32_t orc_round_extract(uint64_t lo, uint64_t hi);
 /* round_extract_gmp2 (src/rlwe_sife.c:334-393): coefficient 0 only, as double. */
 double orc_round_extract_coef0(const uint32_t *d);
 
+/* Key values round_extract(CRT(d_y[.][0])) of every (row, key) pair from
+ * coefficient 0 of the decryption alone (see ipfe_oracle.c).  cl [R][3][N],
+ * c0 [R][l][3] (coefficient 0 of c_0..c_{l-1}), y [S][ly], sk [S][3][N],
+ * kv [R][S].  Returns the number of pairs whose value depends on the INTT's
+ * representative of 0 (their kv is 0xffffffff); 0 when every key ends on a
+ * zero slot. */
+int orc_kvalues_coef0(const uint32_t *cl, const uint32_t *c0, int R, int l, const uint32_t *y, int ly,
+                      const uint32_t *sk, int S, uint32_t *kv);
+
 /* One-vs-one decision (main.c:338-389).  kvalue[l_sv]; sv_coef is
  * [nr_class-1][l_sv] row-major; returns the label.  use_fma selects the
  * contraction gcc applies to "sum += coef * kvalue" at -O3 -march=native. */

@@ -79,6 +79,8 @@ def orc() -> C.CDLL:
         o.orc_round_extract_coef0.argtypes = [_u32p]
         o.orc_svm_decide.restype = C.c_double
         o.orc_svm_decide.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int]
+        o.orc_kvalues_coef0.restype = C.c_int
+        o.orc_kvalues_coef0.argtypes = [_u32p, _u32p, C.c_int, C.c_int, _u32p, C.c_int, _u32p, C.c_int, _u32p]
         o.orc_svm_predict_cpu.restype = C.c_double
         o.orc_svm_predict_cpu.argtypes = [_u32p, C.c_int, _u32p, C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _u32p,
                                           _f64p, C.c_int]
@@ -105,6 +107,23 @@ def ref():
         r.poly_mul_ntt.argtypes = [_u32p, _u32p, _u32p, C.c_uint32]
         _ref = r
     return _ref
+
+
+def kvalues_coef0(cl, c0, y, sk) -> np.ndarray:
+    """kvalue of every (row, key) pair from coefficient 0 of the decryption
+    (orc_kvalues_coef0): cl [R][3][N], c0 [R][l][3], y [S][ly], sk [S][3][N]
+    -> [R][S] uint32.  Raises if a pair depends on the INTT's representative."""
+    cl = np.ascontiguousarray(cl, np.uint32)
+    c0 = np.ascontiguousarray(c0, np.uint32)
+    y = np.ascontiguousarray(y, np.uint32)
+    sk = np.ascontiguousarray(sk, np.uint32)
+    R, l, S, ly = c0.shape[0], c0.shape[1], y.shape[0], y.shape[1]
+    assert cl.shape == (R, 3, N) and sk.shape == (S, 3, N)
+    kv = np.empty((R, S), np.uint32)
+    amb = orc().orc_kvalues_coef0(cl, c0, R, l, y, ly, sk, S, kv)
+    if amb:
+        raise ValueError(f"{amb} pairs end their fold on a non-zero slot")
+    return kv
 
 
 def seed_bytes(s) -> np.ndarray:

@@ -11,8 +11,8 @@ HEADER = os.path.join(ROOT, "include", "cufe_ipfe.h")
 LIB = os.path.join(ROOT, "cufe_amd", "libcufe_ipfe.so")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_\s\*]*?\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text, flags=re.M)
     return sorted({n for n in names if n not in ("defined",)})
@@ -31,6 +31,18 @@ def test_library_exports_every_declared_symbol():
     lib = C.CDLL(LIB)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_bench_library_exports_its_header():
+    """The microbenchmarks live in libcufe_bench.so (include/cufe_bench.h), not
+    in the drop-in library."""
+    bench_lib = os.path.join(ROOT, "cufe_amd", "libcufe_bench.so")
+    if not os.path.exists(bench_lib):
+        pytest.fail("libcufe_bench.so is not built")
+    names = declared_functions(os.path.join(ROOT, "include", "cufe_bench.h"))
+    assert names == ["cufe_microbench"]
+    assert hasattr(C.CDLL(bench_lib), "cufe_microbench")
+    assert not hasattr(C.CDLL(LIB), "cufe_microbench")
 
 
 def test_python_binding_rejects_missing_library(monkeypatch):

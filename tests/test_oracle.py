@@ -324,3 +324,79 @@ def test_svm_decision_matches_libsvm_vote(po):
         k = np.array(kv, np.float64)
         got = po.orc().orc_svm_decide(2, 5, nSV, label, np.ascontiguousarray(coef.ravel()), rho, k, 0)
         assert got == want
+
+
+# A hand-computed three-class model whose two sv_coef rows differ, so that the
+# pair (class 0, class 2) reads different coefficients depending on which row
+# main.c:357-358 takes for which class: coef1 = sv_coef[j-1] on class i's SVs,
+# coef2 = sv_coef[i] on class j's SVs.  Classes 10 / 20 / 30 with 2 / 1 / 2 SVs.
+VOTE3 = {
+    "nSV": [2, 1, 2], "label": [10, 20, 30],
+    "coef": [[1.0, 1.0, -1.0, 2.0, 0.0], [-1.0, -1.0, 1.0, -2.0, 0.5]],
+    "rho": [-0.5, 1.0, 0.0],
+}
+
+
+def vote3_by_hand(kv):
+    """The three pairwise sums written out for VOTE3 (main.c:348-376):
+    (0,1): row0 on class 0 + row0 on class 1; (0,2): row1 on class 0 + row0 on
+    class 2; (1,2): row1 on class 1 + row1 on class 2."""
+    c0, c1 = VOTE3["coef"]
+    rho = VOTE3["rho"]
+    d01 = c0[0] * kv[0] + c0[1] * kv[1] + c0[2] * kv[2] - rho[0]
+    d02 = c1[0] * kv[0] + c1[1] * kv[1] + c0[3] * kv[3] + c0[4] * kv[4] - rho[1]
+    d12 = c1[2] * kv[2] + c1[3] * kv[3] + c1[4] * kv[4] - rho[2]
+    votes = [0, 0, 0]
+    votes[0 if d01 > 0 else 1] += 1
+    votes[0 if d02 > 0 else 2] += 1
+    votes[1 if d12 > 0 else 2] += 1
+    best = max(range(3), key=lambda i: (votes[i], -i))  # first class with most votes
+    return VOTE3["label"][best]
+
+
+def test_svm_vote_three_classes_by_hand(po):
+    """Pins the one-vs-one vote with differing coefficient rows: for
+    kvalues 1..5 the right reading gives votes (2, 0, 1) -> label 10; reading
+    the two rows the other way round gives (1, 0, 2) -> 30."""
+    nSV = np.array(VOTE3["nSV"], np.int32)
+    label = np.array(VOTE3["label"], np.int32)
+    coef = np.ascontiguousarray(np.array(VOTE3["coef"], np.float64).ravel())
+    rho = np.array(VOTE3["rho"], np.float64)
+    # the last two: a three-way tie goes to the first class; class 30 by two votes
+    cases = [([1, 2, 3, 4, 5], 10), ([0, 0, 3, 0, 0], 20), ([0, 0, 0, 1, 0], 10), ([2, 0, 0, 0, 4], 10),
+             ([0, 5, 1, 0, 3], 10), ([3, 0, 0, 0, 0], 30)]
+    for kv, want in cases:
+        assert vote3_by_hand(kv) == want, kv
+        got = po.orc().orc_svm_decide(3, 5, nSV, label, coef, rho, np.array(kv, np.float64), 0)
+        assert got == want, (kv, got)
+    # the swapped reading (coef1 = sv_coef[i], coef2 = sv_coef[j-1]) flips the
+    # (0,2) decision of the first case, so the checks above would catch it
+    c0, c1 = VOTE3["coef"]
+    kv = [1, 2, 3, 4, 5]
+    d02_swapped = c0[0] * kv[0] + c0[1] * kv[1] + c1[3] * kv[3] + c1[4] * kv[4] - VOTE3["rho"][1]
+    assert d02_swapped < 0
+
+
+def test_kvalues_coef0_equals_full_decryption(po):
+    """orc_kvalues_coef0 (coefficient 0 of the decryption only) equals the full
+    RNS decryption followed by round_extract of coefficient 0, for random
+    ciphertexts and keys whose fold ends on zero slots (the row is shorter than
+    the key length), including ciphertext words equal to q."""
+    rng = np.random.default_rng(11)
+    l, ly, R, S = 5, 8, 3, 4
+    qs = np.array(Q, np.uint64)[None, :, None]
+    c = (rng.integers(0, 1 << 62, (R, ly + 1, 3, N), dtype=np.uint64) % qs).astype(np.uint32)
+    c[0, 1, 2, 0] = Q[2]  # a coefficient-0 word equal to q
+    c[1, ly, 0, 17] = Q[0]  # a c_L word equal to q
+    msk = np.empty((ly, 3, N), np.uint32)
+    z = rng.integers(-900, 901, (ly, N))
+    for j in range(3):
+        msk[:, j, :] = np.where(z < 0, Q[j] + z, z).astype(np.uint32)
+    y = np.zeros((S, ly), np.uint32)
+    y[:, :l] = rng.integers(0, 17, (S, l))
+    sk = np.stack([po.keygen(y[v], msk) for v in range(S)])
+    kv = po.kvalues_coef0(c[:, ly], c[:, :l, :, 0], y, sk)
+    for r in range(R):
+        for v in range(S):
+            d = po.decrypt(c[r], y[v], sk[v])
+            assert kv[r, v] == int(po.orc().orc_round_extract_coef0(np.ascontiguousarray(d))), (r, v)

@@ -15,7 +15,9 @@ the rows and functional-key generation are timed separately and reported.
 Extra fields: the plain per-call GPU form (the four reference C-ABI calls per
 row, svm_predict_gpu), the roofline of the dominant kernel (the int8
 matrix-core contraction, timed with HIP events on its stream; the fp64
-contraction when keys need it), and the CPU oracle on a bounded sample.
+contraction when keys need it), and svm_predict_cpu on a bounded sample: the
+reference's own C path compiled from its sources (oracle/_ref) where that build
+is present, else the oracle's restatement of it.
 """
 from __future__ import annotations
 
@@ -53,66 +55,76 @@ def sparse_binary_rows(rng, n, dim=123, ones=14, width=124):
     return x
 
 
-def cpu_baseline(cf, ctx, model, x_row, seconds: float):
-    """svm_predict_cpu (main.c:137-267) restated in oracle/ipfe_oracle.c, one
-    core, on the first k support vectors of one row; extrapolated to a full
-    prediction (all SVs)."""
+def _svm_predict_cpu_fn():
+    """svm_predict_cpu (main.c:137-267) for one ciphertext against k SVs: the
+    reference's own compiled C path (oracle/_ref/libref.so, built from the
+    reference sources with GMP) when it is present, else the oracle's
+    restatement.  Returns (kind, fn(c, msk, sv))."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
+    ref = po.ref()
+
+    def run(c, msk, sv):
+        k = sv.shape[0]
+        nSV, lab = np.array([k, 0], np.int32), np.array([1, -1], np.int32)
+        coef, rho = np.ascontiguousarray(np.ones(k)), np.array([0.0])
+        if ref is not None:
+            return ref.refh_svm_predict_cpu(c, msk, 2, k, nSV, lab, coef, rho, sv)
+        return po.orc().orc_svm_predict_cpu(c, msk.shape[0], msk, 2, k, nSV, lab, coef, rho, sv, np.empty(k), 0)
+
+    return ("reference" if ref is not None else "port"), run
+
+
+def cpu_baseline(cf, ctx, model, x_row, seconds: float):
+    """svm_predict_cpu on one core, on blocks of 4 support vectors of one row
+    (keygen, full decryption with GMP CRT, rounding of all 4096 coefficients
+    per SV) for about `seconds`; extrapolated to a full prediction (all SVs)."""
+    kind, run = _svm_predict_cpu_fn()
     mpk, msk = ctx.get_keys()
     L = msk.shape[0]
     xf = np.zeros((1, L), np.uint32)
     xf[0, : x_row.size] = x_row
     c = cf.encrypt(xf, mpk, h(b"cpu2"), h(b"cpu3"))[0]  # bit-identical to the CPU encryption
     S = model.l
-    kv = np.empty(S)
     done = 0
     t0 = time.perf_counter()
     while done < S and time.perf_counter() - t0 < seconds:
         k = min(4, S - done)
-        sv = np.ascontiguousarray(model.sv[done: done + k])
-        coef = np.ascontiguousarray(np.ones(k))
-        kvk = np.empty(k)
-        po.orc().orc_svm_predict_cpu(c, L, msk, 2, k, np.array([k, 0], np.int32), np.array([1, -1], np.int32),
-                                     coef, np.array([0.0]), sv, kvk, 0)
-        kv[done: done + k] = kvk
+        run(c, msk, np.ascontiguousarray(model.sv[done: done + k]))
         done += k
     dt = time.perf_counter() - t0
     per_pair = dt / done
-    return {"value": 1.0 / (per_pair * S), "unit": "predictions/s", "cores": 1, "kind": "port",
-            "sample": f"1 row x {done} of {S} SVs in {dt:.1f} s (dense keygen + decrypt + CRT of all 4096 "
-                      f"coefficients per SV, as svm_predict_cpu); {per_pair * 1e3:.1f} ms per (row, SV) pair, "
-                      f"extrapolated to {S} SVs"}
+    what = "reference build of svm_predict_cpu (gcc -O3 -DAVX2, GMP)" if kind == "reference" else \
+        "oracle restatement of svm_predict_cpu"
+    return {"value": 1.0 / (per_pair * S), "unit": "predictions/s", "cores": 1, "kind": kind,
+            "sample": f"1 row x {done} of {S} SVs in {dt:.1f} s, {what}; {per_pair * 1e3:.1f} ms per (row, SV) "
+                      f"pair, extrapolated to {S} SVs"}
 
 
 def _cpu_worker(args):
-    """One spawned process: times the oracle's svm_predict_cpu restatement on
-    support vectors [v0, v0 + k) repeatedly for `seconds`; returns pairs done.
-    It loads only the CPU oracle (never the HIP runtime)."""
+    """One spawned process: times svm_predict_cpu on support vectors
+    [v0, v0 + k) repeatedly for `seconds`; returns pairs done.  It loads only
+    the CPU libraries (never the HIP runtime)."""
     path, v0, k, seconds = args
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as po
-
+    _, run = _svm_predict_cpu_fn()
     d = np.load(path)
     c, msk, sv = d["c"], d["msk"], np.ascontiguousarray(d["sv"][v0:v0 + k])
-    L = msk.shape[0]
-    kvk = np.empty(k)
     done = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        po.orc().orc_svm_predict_cpu(c, L, msk, 2, k, np.array([k, 0], np.int32), np.array([1, -1], np.int32),
-                                     np.ascontiguousarray(np.ones(k)), np.array([0.0]), sv, kvk, 0)
+        run(c, msk, sv)
         done += k
     return done, time.perf_counter() - t0
 
 
 def cpu_baseline_all_cores(cf, ctx, model, x_row, seconds: float):
-    """The same CPU restatement on every host core the box grants (at most 16),
+    """The same svm_predict_cpu on every host core the box grants (at most 16),
     one spawned process per core, each on its own slice of the support vectors."""
     import multiprocessing as mp
     import tempfile
 
+    kind, _ = _svm_predict_cpu_fn()
     mpk, msk = ctx.get_keys()
     L = msk.shape[0]
     xf = np.zeros((1, L), np.uint32)
@@ -134,7 +146,7 @@ def cpu_baseline_all_cores(cf, ctx, model, x_row, seconds: float):
     pairs = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
     rate_pairs = pairs / wall
-    return {"value": rate_pairs / model.l, "unit": "predictions/s", "cores": cores, "kind": "port",
+    return {"value": rate_pairs / model.l, "unit": "predictions/s", "cores": cores, "kind": kind,
             "sample": f"{cores} processes x {seconds:.0f} s, {pairs} (row, SV) pairs of svm_predict_cpu "
                       f"({rate_pairs:.1f} pairs/s), extrapolated to {model.l} SVs per prediction"}
 
@@ -537,9 +549,9 @@ def main():
         out["serving"]["batched_after_other_legs"] = sb.run("batched_after_other_legs", True, 1)
         sb.close()
     if not args.no_cpu:
-        log("cpu baseline (oracle, 1 core)")
+        log("cpu baseline (svm_predict_cpu, 1 core)")
         out["cpu_baseline"] = cpu_baseline(cf, ctx, model, x[0], args.cpu_seconds)
-        log("cpu baseline (oracle, all granted cores)")
+        log("cpu baseline (svm_predict_cpu, all granted cores)")
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(cf, ctx, model, x[0], args.cpu_seconds / 2)
     ctx.close()
     out["autotune"] = cf.autotune_report().strip().split("\n")  # every shape tuned, serving batches included

@@ -3,9 +3,9 @@ tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
 product (cufe_amd).
 
   liboracle.so   -- ipfe_oracle.c, the CPU restatement of the reference path
-  _ref/libref.so -- the reference's own GMP-free sources (aes256ctr.c,
-                    arith_rns.c, gauss.c, ntt.c) compiled in this tree; absent
-                    where the reference is not mounted (e.g. the GPU host)
+  _ref/libref.so -- the reference's own C path (src/*.c, main.c) compiled
+                    against GMP, plus ref_harness.c; built where the
+                    reference is mounted (it travels with the tree once built)
 """
 from __future__ import annotations
 
@@ -92,7 +92,17 @@ def ref():
     """The reference's own compiled subset, or None when it is not built here."""
     global _ref
     if _ref is None and os.path.exists(REF_SO):
-        r = C.CDLL(REF_SO)
+        # lazy binding: the reference functions that call its CUDA module
+        # (never called here) keep their symbols unresolved
+        r = C.CDLL(REF_SO, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        r.sample_polya.argtypes = [_u8p, _u32p]
+        r.round_extract_gmp2.restype = C.c_double
+        r.round_extract_gmp2.argtypes = [_u32p]
+        r.rlwe_sife_keygen.argtypes = [_u32p, _u32p, _u32p, C.POINTER(C.c_double)]
+        r.refh_crt_round.argtypes = [_u32p, _u32p, _u32p]
+        r.refh_decrypt_round.argtypes = [_u32p, _u32p, _u32p, _u32p]
+        r.refh_svm_predict_cpu.restype = C.c_double
+        r.refh_svm_predict_cpu.argtypes = [_u32p, _u32p, C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _u32p]
         r.mod_prime.restype = C.c_uint32
         r.mod_prime.argtypes = [C.c_uint64, C.c_uint32]
         r.aes256ctr_init.argtypes = [C.c_void_p, _u8p, C.c_uint16]

@@ -31,6 +31,12 @@ extern "C" const char *cufe_last_error(void) { return ipfe::t_err; }
 
 namespace ipfe {
 
+// The constant tables live in static device arrays of this file, so that the
+// library's set-up allocates no device memory.  Host code reaches them only
+// through the addresses hipGetSymbolAddress returns.
+__device__ __align__(256) uint32_t d_tw_all_tab[4 * 3 * IPFE_N];
+__device__ __align__(256) uint32_t d_aes_te_tab[1024];
+
 static Device *init_device()
 {
     const HostTables &ht = host_tables();
@@ -40,13 +46,16 @@ static Device *init_device()
         ipfe_set_error("no HIP device available");
         return nullptr;
     }
-    if (hipMalloc((void **)&dev.tw_all, ht.tw_all.size() * 4) != hipSuccess ||
-        hipMalloc((void **)&dev.aes_te, ht.aes_te.size() * 4) != hipSuccess) {
-        ipfe_set_error("hipMalloc of constant tables failed");
+    if (hipGetDevice(&dev.id) != hipSuccess ||
+        hipGetSymbolAddress((void **)&dev.tw_all, HIP_SYMBOL(d_tw_all_tab)) != hipSuccess ||
+        hipGetSymbolAddress((void **)&dev.aes_te, HIP_SYMBOL(d_aes_te_tab)) != hipSuccess) {
+        ipfe_set_error("device addresses of the constant tables unavailable");
         return nullptr;
     }
-    if (hipMemcpy(dev.tw_all, ht.tw_all.data(), ht.tw_all.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dev.aes_te, ht.aes_te.data(), ht.aes_te.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    static_assert(sizeof(d_tw_all_tab) == 4 * 3 * IPFE_N * 4 && sizeof(d_aes_te_tab) == 1024 * 4, "table sizes");
+    if (ht.tw_all.size() * 4 != sizeof(d_tw_all_tab) || ht.aes_te.size() * 4 != sizeof(d_aes_te_tab) ||
+        hipMemcpy(dev.tw_all, ht.tw_all.data(), sizeof(d_tw_all_tab), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dev.aes_te, ht.aes_te.data(), sizeof(d_aes_te_tab), hipMemcpyHostToDevice) != hipSuccess) {
         ipfe_set_error("upload of constant tables failed");
         return nullptr;
     }
@@ -116,7 +125,7 @@ size_t gauss_bytes(int which, int ncalls, int coef0_only)
     return coef0_only ? (size_t)ch * 1 + (size_t)ncalls * 8 : (size_t)ch * (1 + 64 + 4);
 }
 
-int GaussPlan::build(const StreamReq *reqs, int count, int lvl)
+int GaussPlan::build(const StreamReq *reqs, int count, int lvl, bool alloc_meta)
 {
     constexpr int kTile = GAUSS_TILE_CHUNKS;
     n = count;
@@ -155,10 +164,55 @@ int GaussPlan::build(const StreamReq *reqs, int count, int lvl)
         if (g.coef0_only) nfirst += g.ncalls;
         else full_chunks = chunks;
     }
+    if (!alloc_meta) return 0;
     if (d_streams.alloc(n) || d_tp.alloc(n) || d_cp.alloc(n) || d_fp.alloc(n) || d_status.alloc(n + 1) ||
         d_first.alloc((size_t)nfirst + 1))
         return -1;
+    m_streams = d_streams.get();
+    m_tp = d_tp.get();
+    m_cp = d_cp.get();
+    m_fp = d_fp.get();
+    m_first = d_first.get();
+    m_status = d_status.get();
     return 0;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t GaussPlan::meta_bytes() const
+{
+    return align256(sizeof(GaussStream) * n) + 3 * align256(sizeof(long long) * n) +
+           align256(sizeof(long long) * ((size_t)nfirst + 1)) + align256(sizeof(int) * ((size_t)n + 1));
+}
+
+int GaussPlan::bind_meta(uint8_t *base)
+{
+    m_streams = (GaussStream *)base;
+    base += align256(sizeof(GaussStream) * n);
+    m_tp = (long long *)base;
+    base += align256(sizeof(long long) * n);
+    m_cp = (long long *)base;
+    base += align256(sizeof(long long) * n);
+    m_fp = (long long *)base;
+    base += align256(sizeof(long long) * n);
+    m_first = (long long *)base;
+    base += align256(sizeof(long long) * ((size_t)nfirst + 1));
+    m_status = (int *)base;
+    return 0;
+}
+
+size_t GaussPlan::bound_scratch_bytes() const
+{
+    return align256((size_t)chunks) + align256((size_t)full_chunks * 64 + 8) + align256((size_t)full_chunks * 4 + 4);
+}
+
+int GaussPlan::bind_scratch_region(uint8_t *base)
+{
+    uint8_t *cnt = base;
+    base += align256((size_t)chunks);
+    uint64_t *vals = (uint64_t *)base;
+    base += align256((size_t)full_chunks * 64 + 8);
+    return bind_scratch(cnt, vals, (int *)base);
 }
 
 size_t GaussPlan::scratch_bytes() const { return (size_t)chunks + (size_t)full_chunks * (64 + 4) + 64; }
@@ -195,31 +249,28 @@ void GaussPlan::rekey(const StreamReq *reqs)
 
 int GaussPlan::upload(hipStream_t st)
 {
-    HIP_OK(hipMemcpyAsync(d_streams.get(), hs.data(), sizeof(GaussStream) * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(d_tp.get(), tile_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(d_cp.get(), chunk_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(d_fp.get(), first_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(m_streams, hs.data(), sizeof(GaussStream) * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(m_tp, tile_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(m_cp, chunk_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(m_fp, first_prefix.data(), sizeof(long long) * n, hipMemcpyHostToDevice, st));
     return 0;
 }
 
 // Kernels only (fills, candidates, walk, scatter, firsts): capturable.
 int GaussPlan::launch(hipStream_t st)
 {
-    if (launch_fill_i32(s_pos, -1, (size_t)full_chunks, st) || launch_fill_i32(d_status.get(), 0, n + 1, st))
-        return -1;
-    if (launch_gauss_candidates(d_streams.get(), n, d_tp.get(), tiles, s_cnt, s_vals, d_status.get() + n, full, st))
-        return -1;
-    if (launch_gauss_walk(d_streams.get(), n, s_cnt, s_pos, d_first.get(), d_status.get(), st)) return -1;
-    if (launch_gauss_scatter(d_streams.get(), n, d_cp.get(), full_chunks, s_cnt, s_vals, s_pos, st)) return -1;
-    if (launch_gauss_firsts(d_streams.get(), n, d_fp.get(), d_first.get(), nfirst, d_status.get() + n, full, st))
-        return -1;
+    if (launch_fill_i32(s_pos, -1, (size_t)full_chunks, st) || launch_fill_i32(m_status, 0, n + 1, st)) return -1;
+    if (launch_gauss_candidates(m_streams, n, m_tp, tiles, s_cnt, s_vals, m_status + n, full, st)) return -1;
+    if (launch_gauss_walk(m_streams, n, s_cnt, s_pos, m_first, m_status, st)) return -1;
+    if (launch_gauss_scatter(m_streams, n, m_cp, full_chunks, s_cnt, s_vals, s_pos, st)) return -1;
+    if (launch_gauss_firsts(m_streams, n, m_fp, m_first, nfirst, m_status + n, full, st)) return -1;
     return 0;
 }
 
 int GaussPlan::check(hipStream_t st, int *overflow, int *need_full)
 {
     std::vector<int> status(n + 1);
-    HIP_OK(hipMemcpyAsync(status.data(), d_status.get(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(status.data(), m_status, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     if (status[n] & 2) {
         ipfe_set_error("Gaussian sampler: a chunk needed more than 46 uniform words (reference reads past its buffer)");

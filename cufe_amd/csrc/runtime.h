@@ -13,8 +13,11 @@ namespace ipfe {
 
 extern const uint32_t *g_aes_te;
 
-// Per-process device state (one GPU): constant tables resident in HBM.
+// Per-process device state (one GPU, the one current at the library's first
+// call): constant tables resident in HBM, in static device arrays (set-up
+// allocates nothing).
 struct Device {
+    int id = 0;                  // the HIP device the state belongs to
     uint32_t *tw_all = nullptr;  // [4][3][N]
     uint32_t *aes_te = nullptr;  // [1024]
     NinvConsts ninv;
@@ -85,6 +88,11 @@ struct GaussPlan {
     DevBuf<GaussStream> d_streams;
     DevBuf<long long> d_tp, d_cp, d_fp, d_first;
     DevBuf<int> d_status;
+    // device metadata the kernels read: the buffers above, or a region bound
+    // with bind_meta
+    GaussStream *m_streams = nullptr;
+    long long *m_tp = nullptr, *m_cp = nullptr, *m_fp = nullptr, *m_first = nullptr;
+    int *m_status = nullptr;
     DevBuf<uint8_t> own_cnt;
     DevBuf<uint64_t> own_vals;
     DevBuf<int> own_pos;
@@ -92,7 +100,14 @@ struct GaussPlan {
     uint64_t *s_vals = nullptr;
     int *s_pos = nullptr;
 
-    int build(const StreamReq *reqs, int n, int level);
+    // alloc_meta false: plan only; the caller binds the device metadata
+    // (bind_meta) and the chunk scratch (bind_scratch_region) to its memory
+    int build(const StreamReq *reqs, int n, int level, bool alloc_meta = true);
+    size_t meta_bytes() const;
+    int bind_meta(uint8_t *base);
+    // the chunk scratch as one region of 256-byte aligned pieces
+    size_t bound_scratch_bytes() const;
+    int bind_scratch_region(uint8_t *base);
     // chunk scratch: own allocation, or a caller's buffer of scratch_bytes()
     // (chunks counts, then 8 values and 1 position per full-stream chunk)
     size_t scratch_bytes() const;

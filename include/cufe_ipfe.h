@@ -121,6 +121,20 @@ int cufe_gauss_sample(const unsigned char *keys, const uint16_t *nonces, int nst
  * coefficient 0. */
 int cufe_gauss_sample_first(const unsigned char *keys, const uint16_t *nonces, int nstreams, int which, int ncalls,
                             uint32_t *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cufe_gauss_sample with the output in DEVICE memory the
+ * caller owns, d_out [nstreams][ncalls][3][4096], and every scratch buffer of
+ * the sampler in one static 64 MiB device array of the library (held per GPU
+ * until the process ends).  Nothing is allocated: a single stream whose
+ * speculative window does not fit the array (about 600 calls of sampler 3,
+ * more for 1 and 2) is rejected with an error; more streams run in groups.
+ * Calls on one GPU share the array and are serialised within the process;
+ * calls that overlap on one GPU from elsewhere would race.  Runs on the
+ * device the library was first used on, on the default stream, and returns
+ * after the samples are written.  0 or -1 (reason in cufe_last_error()). */
+int cufe_gauss_sample_noalloc(const unsigned char *keys, const uint16_t *nonces, int nstreams, int which, int ncalls,
+                              uint32_t *d_out);
 int cufe_aes256ctr(const unsigned char key[32], uint16_t nonce, uint32_t ctr0, uint32_t nblocks, unsigned char *out);
 /* CRT + rounding of coefficient coef of nvec RNS polys d[nvec][3][4096]. */
 int cufe_crt_round(const uint32_t *d, int nvec, int coef, uint32_t *r);

@@ -355,6 +355,142 @@ def time_gauss(nstreams: int, which: int, iters: int) -> float:
 
 
 # ---- resident context and batched prediction -------------------------------------
+# ---- parameter sets -------------------------------------------------------------
+def _ps_sigs(lb):
+    fp = C.POINTER(C.c_float)
+    sig = {
+        "cufe_pset_info": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "cufe_ps_ntt": (C.c_int, [C.c_int, C.c_int, _u32p, C.c_int]),
+        "cufe_ps_ntt_variant": (C.c_int, [C.c_int, C.c_int, C.c_int, _u32p, C.c_int]),
+        "cufe_ntt_variants": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
+        "cufe_ntt_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
+        "cufe_ps_sample_polya": (C.c_int, [C.c_int, _u8p, _u32p]),
+        "cufe_ps_gauss_sample": (C.c_int, [C.c_int, _u8p, _u16p, C.c_int, C.c_int, C.c_int, _u32p]),
+        "cufe_ps_setup": (C.c_int, [C.c_int, _u32p, _u32p, _u8p, _u8p, C.c_int]),
+        "cufe_ps_encrypt": (C.c_int, [C.c_int, _u32p, _u32p, _u32p, _u8p, _u8p, C.c_int, C.c_int, C.c_int]),
+        "cufe_ps_keygen": (C.c_int, [C.c_int, _u32p, _u32p, _u32p, C.c_int, C.c_int]),
+        "cufe_ps_decrypt": (C.c_int, [C.c_int, _u32p, _u32p, _u32p, _u32p, C.c_int, C.c_int]),
+        "cufe_ps_crt_round": (C.c_int, [C.c_int, _u32p, C.c_int, C.c_int, _u32p]),
+        "cufe_ps_crt_round_host": (C.c_int, [C.c_int, _u32p, C.POINTER(C.c_uint32)]),
+        "cufe_time_ps_ntt": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, fp]),
+        "cufe_time_ps_gauss": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, fp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lb, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _pslib():
+    lb = lib()
+    if not getattr(lb, "_ps_sigs", False):
+        _ps_sigs(lb)
+        lb._ps_sigs = True
+    return lb
+
+
+class PSet:
+    """One of the library's parameter sets: 0 (N = 2048, 2 limbs), 1 (the
+    reference's N = 4096, 3 limbs), 2 (N = 8192, 4 limbs).  Arrays are
+    [..][nmod][N] as for the reference set."""
+
+    def __init__(self, pset: int):
+        n, nm, l = C.c_int(), C.c_int(), C.c_int()
+        q = (C.c_uint32 * 4)()
+        p = C.c_uint32()
+        _check(_pslib().cufe_pset_info(pset, C.byref(n), C.byref(nm), C.byref(l), q, C.byref(p)), "pset_info")
+        self.id, self.N, self.nmod, self.L, self.p = pset, n.value, nm.value, l.value, p.value
+        self.q = tuple(int(q[j]) for j in range(self.nmod))
+
+    def ntt(self, polys, inverse: bool = False, variant: int | None = None) -> np.ndarray:
+        a = np.array(polys, dtype=np.uint32, copy=True, order="C")
+        npoly = a.size // (self.nmod * self.N)
+        assert a.size == npoly * self.nmod * self.N
+        if variant is None:
+            _check(_pslib().cufe_ps_ntt(self.id, int(inverse), a, npoly), "ps_ntt")
+        else:
+            _check(_pslib().cufe_ps_ntt_variant(self.id, variant, int(inverse), a, npoly), "ps_ntt_variant")
+        return a
+
+    def ntt_variants(self) -> list:
+        buf = C.create_string_buffer(4096)
+        n = _pslib().cufe_ntt_variants(self.id, buf, 4096)
+        _check(0 if n >= 0 else -1, "ntt_variants")
+        return buf.value.decode().splitlines()[:n]
+
+    def sample_polya(self, seed) -> np.ndarray:
+        a = np.empty((self.nmod, self.N), np.uint32)
+        _check(_pslib().cufe_ps_sample_polya(self.id, _seed(seed), a), "ps_sample_polya")
+        return a
+
+    def gauss_sample(self, keys, nonces, which: int, ncalls: int) -> np.ndarray:
+        k = np.ascontiguousarray(np.frombuffer(b"".join(bytes(x) for x in keys), dtype=np.uint8))
+        n = np.ascontiguousarray(nonces, dtype=np.uint16)
+        out = np.empty((len(n), ncalls, self.nmod, self.N), np.uint32)
+        _check(_pslib().cufe_ps_gauss_sample(self.id, k, n, len(n), which, ncalls, out), "ps_gauss_sample")
+        return out
+
+    def setup(self, a_coeff, seed2, seed3, l: int):
+        mpk = np.zeros((l + 1, self.nmod, self.N), np.uint32)
+        mpk[l] = _u32(a_coeff, (self.nmod, self.N))
+        msk = np.zeros((l, self.nmod, self.N), np.uint32)
+        _check(_pslib().cufe_ps_setup(self.id, mpk, msk, _seed(seed2), _seed(seed3), l), "ps_setup")
+        return mpk, msk
+
+    def encrypt(self, m, mpk, seed2, seed3, nonce0: int = 0) -> np.ndarray:
+        m = _u32(np.atleast_2d(m))
+        repeat, l = m.shape
+        mpk = _u32(mpk)
+        assert mpk.shape == (l + 1, self.nmod, self.N)
+        c = np.empty((repeat, l + 1, self.nmod, self.N), np.uint32)
+        _check(_pslib().cufe_ps_encrypt(self.id, m if m.size else np.zeros(1, np.uint32), mpk, c, _seed(seed2),
+                                        _seed(seed3), repeat, l, nonce0), "ps_encrypt")
+        return c
+
+    def keygen(self, y, msk) -> np.ndarray:
+        y = _u32(np.atleast_2d(y))
+        repeat, l = y.shape
+        sk = np.empty((repeat, self.nmod, self.N), np.uint32)
+        _check(_pslib().cufe_ps_keygen(self.id, y, _u32(msk), sk, repeat, l), "ps_keygen")
+        return sk
+
+    def decrypt(self, c, y, sk) -> np.ndarray:
+        y = _u32(np.atleast_2d(y))
+        repeat, l = y.shape
+        d = np.empty((repeat, self.nmod, self.N), np.uint32)
+        _check(_pslib().cufe_ps_decrypt(self.id, _u32(c), y, _u32(sk), d, repeat, l), "ps_decrypt")
+        return d
+
+    def crt_round(self, d, coef: int = 0) -> np.ndarray:
+        d = _u32(d)
+        nvec = d.size // (self.nmod * self.N)
+        r = np.empty((nvec,) if coef >= 0 else (nvec, self.N), np.uint32)
+        _check(_pslib().cufe_ps_crt_round(self.id, d, nvec, coef, r), "ps_crt_round")
+        return r
+
+    def crt_round_host(self, res) -> int:
+        r = C.c_uint32()
+        _check(_pslib().cufe_ps_crt_round_host(self.id, _u32(res), C.byref(r)), "ps_crt_round_host")
+        return r.value
+
+    def time_ntt(self, npoly: int, inverse: bool, iters: int) -> float:
+        ms = C.c_float()
+        _check(_pslib().cufe_time_ps_ntt(self.id, npoly, int(inverse), iters, C.byref(ms)), "time_ps_ntt")
+        return ms.value
+
+    def time_gauss(self, nstreams: int, which: int, iters: int) -> float:
+        ms = C.c_float()
+        _check(_pslib().cufe_time_ps_gauss(self.id, nstreams, which, iters, C.byref(ms)), "time_ps_gauss")
+        return ms.value
+
+
+def ntt_autotune_report() -> str:
+    buf = C.create_string_buffer(1 << 16)
+    _pslib().cufe_ntt_autotune_report(buf, 1 << 16)
+    return buf.value.decode()
+
+
 class SvmModel:
     """Plain linear SVM model: the fields svm_predict_gpu reads (svm/svm.h:52-72),
     support vectors dense [l][785] (feature index = slot)."""

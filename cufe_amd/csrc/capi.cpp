@@ -9,6 +9,7 @@
 #include "../../include/cufe_ipfe.h"
 #include "crt_dev.h"
 #include "runtime.h"
+#include "kernels_pset.h"
 
 using namespace ipfe;
 
@@ -356,7 +357,8 @@ extern "C" int cufe_ntt_forward(uint32_t *polys, int npoly)
     DevBuf<uint32_t> d;
     if (d.alloc((size_t)npoly * kPoly)) return -1;
     HIP_OK(hipMemcpyAsync(d.get(), polys, (size_t)npoly * kPoly * 4, hipMemcpyHostToDevice, st));
-    if (launch_ntt_fwd(d.get(), d.get(), npoly * 3, dev->tw_all, st)) return -1;
+    const PsetState *ps = pset_state(1);  // the reference set, tuned transform shape
+    if (!ps || ps_ntt(*ps, false, d.get(), d.get(), npoly * 3, st)) return -1;
     HIP_OK(hipMemcpyAsync(polys, d.get(), (size_t)npoly * kPoly * 4, hipMemcpyDeviceToHost, st));
     return sync_stream(st);
 }
@@ -369,7 +371,8 @@ extern "C" int cufe_ntt_inverse(uint32_t *polys, int npoly)
     DevBuf<uint32_t> d;
     if (d.alloc((size_t)npoly * kPoly)) return -1;
     HIP_OK(hipMemcpyAsync(d.get(), polys, (size_t)npoly * kPoly * 4, hipMemcpyHostToDevice, st));
-    if (launch_ntt_inv(d.get(), d.get(), npoly * 3, dev->tw_all, dev->ninv, st)) return -1;
+    const PsetState *ps = pset_state(1);
+    if (!ps || ps_ntt(*ps, true, d.get(), d.get(), npoly * 3, st)) return -1;
     HIP_OK(hipMemcpyAsync(polys, d.get(), (size_t)npoly * kPoly * 4, hipMemcpyDeviceToHost, st));
     return sync_stream(st);
 }
@@ -465,15 +468,14 @@ extern "C" int cufe_time_ntt(int npoly, int inverse, int iters, float *ms_per_la
     if (d.alloc((size_t)npoly * kPoly)) return -1;
     // synthetic residues: any fixed pattern in [0, q0)
     HIP_OK(hipMemsetD32Async((hipDeviceptr_t)d.get(), 65537u, (size_t)npoly * kPoly, st));
+    // the reference set's tuned transform shape (tuned on first use)
+    const PsetState *ps = pset_state(1);
+    if (!ps) return -1;
     for (int w = 0; w < 2; ++w)
-        if (inverse ? launch_ntt_inv(d.get(), d.get(), npoly * 3, dev->tw_all, dev->ninv, st)
-                    : launch_ntt_fwd(d.get(), d.get(), npoly * 3, dev->tw_all, st))
-            return -1;
+        if (ps_ntt(*ps, inverse != 0, d.get(), d.get(), npoly * 3, st)) return -1;
     EventTimer tm(st);
     for (int it = 0; it < iters; ++it)
-        if (inverse ? launch_ntt_inv(d.get(), d.get(), npoly * 3, dev->tw_all, dev->ninv, st)
-                    : launch_ntt_fwd(d.get(), d.get(), npoly * 3, dev->tw_all, st))
-            return -1;
+        if (ps_ntt(*ps, inverse != 0, d.get(), d.get(), npoly * 3, st)) return -1;
     float e = tm.stop();
     if (ms_per_launch) *ms_per_launch = e / (iters > 0 ? iters : 1);
     return sync_stream(st);

@@ -49,6 +49,19 @@ __host__ __device__ inline uint32_t mod_prime_exact(uint64_t m, int s)
     return (uint32_t)m;
 }
 
+// mod_prime for any prime q = 2^a - 2^b + 1 (the other parameter sets).
+__host__ __device__ inline uint32_t mod_prime_gen(uint64_t m, uint32_t q32, uint32_t a, uint32_t b)
+{
+    const uint64_t q = q32;
+    const uint64_t lo = ((uint64_t)1 << a) - 1;
+    while (m > 2 * q) {
+        uint64_t hi = m >> a;
+        m = (m & lo) + ((hi << b) - hi);
+    }
+    if (m >= q) m -= q;
+    return (uint32_t)m;
+}
+
 // add_mod_ntt / sub_mod_ntt (src/arith_rns.c:22-48).  Operands lie in [0, q]
 // with q < 2^31, so the 32-bit sums cannot overflow.
 __host__ __device__ inline uint32_t add_q(uint32_t a, uint32_t b, uint32_t q)

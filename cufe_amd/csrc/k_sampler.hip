@@ -27,13 +27,33 @@ namespace ipfe {
 
 // Sampler constants (src/gauss.c:11-45): k per width, the Barrett factor
 // floor(2^32/k), the uniform acceptance bound k*floor(2^32/k) and -1/k^2.
-__constant__ uint64_t c_K[4] = {1ull, 266ull, 304214978ull, 608429953ull};
-__constant__ uint64_t c_BF[4] = {0ull, (1ull << 32) / 266ull, (1ull << 32) / 304214978ull, (1ull << 32) / 608429953ull};
-__constant__ uint64_t c_UQ[4] = {0ull, 266ull * ((1ull << 32) / 266ull), 304214978ull * ((1ull << 32) / 304214978ull),
-                                 608429953ull * ((1ull << 32) / 608429953ull)};
-__constant__ double c_K2INV[4] = {0.0, -1.0 / (double)(long long)(266ll * 266ll),
-                                  -1.0 / (double)(long long)(304214978ll * 304214978ll),
-                                  -1.0 / (double)(long long)(608429953ll * 608429953ll)};
+// Widths 1-3 are the reference's S1/S2/S3; 4 and 5 are the S2/S3 widths of the
+// N = 2048 parameter set (the same algorithm with another k).
+#define IPFE_SAMPLER_K {1ull, 266ull, 304214978ull, 608429953ull, 262147ull, 524287ull}
+constexpr uint64_t kSamplerK[6] = IPFE_SAMPLER_K;
+__constant__ uint64_t c_K[6] = IPFE_SAMPLER_K;
+__constant__ uint64_t c_BF[6] = {0ull, (1ull << 32) / kSamplerK[1], (1ull << 32) / kSamplerK[2],
+                                 (1ull << 32) / kSamplerK[3], (1ull << 32) / kSamplerK[4],
+                                 (1ull << 32) / kSamplerK[5]};
+__constant__ uint64_t c_UQ[6] = {0ull, kSamplerK[1] * ((1ull << 32) / kSamplerK[1]),
+                                 kSamplerK[2] * ((1ull << 32) / kSamplerK[2]),
+                                 kSamplerK[3] * ((1ull << 32) / kSamplerK[3]),
+                                 kSamplerK[4] * ((1ull << 32) / kSamplerK[4]),
+                                 kSamplerK[5] * ((1ull << 32) / kSamplerK[5])};
+__constant__ double c_K2INV[6] = {0.0, -1.0 / (double)(long long)(kSamplerK[1] * kSamplerK[1]),
+                                  -1.0 / (double)(long long)(kSamplerK[2] * kSamplerK[2]),
+                                  -1.0 / (double)(long long)(kSamplerK[3] * kSamplerK[3]),
+                                  -1.0 / (double)(long long)(kSamplerK[4] * kSamplerK[4]),
+                                  -1.0 / (double)(long long)(kSamplerK[5] * kSamplerK[5])};
+
+// Residue of a candidate on limb s of the stream's parameter set (src/gauss.c:773-779).
+__device__ __forceinline__ uint32_t stream_residue(const GaussStream *S, int s, uint64_t z, bool neg)
+{
+    const uint32_t q = S->q[s], a = S->fold[s] & 255u, b = S->fold[s] >> 8;
+    uint32_t m = mod_prime_gen(z, q, a, b);
+    if (neg) m = mod_prime_gen((uint32_t)(q - m), q, a, b);
+    return m;
+}
 
 // CDT rows (low word, high word), src/gauss.c:62-72.
 __constant__ uint64_t c_CDT[9][2] = {{2200310400551559144ull, 3327841033070651387ull},
@@ -392,13 +412,8 @@ __global__ __launch_bounds__(CAND_THREADS) void k_gauss_firsts(const GaussStream
     if (valid && cd.acc && (gm & ((1u << c) - 1u)) == 0) {
         const GaussStream *S = streams + si;
         const long long call = t0 + k - first_prefix[si];
-        uint32_t *out = S->out0 + 3 * call;
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            uint32_t m = mod_prime_exact(cd.z, s);
-            if (cd.sgn) m = mod_prime_exact((uint32_t)(qmod(s) - m), s);
-            out[s] = m;
-        }
+        uint32_t *out = S->out0 + (long long)S->nmod * call;
+        for (int s = 0; s < S->nmod; ++s) out[s] = stream_residue(S, s, cd.z, cd.sgn != 0);
     }
 }
 
@@ -450,7 +465,7 @@ __global__ __launch_bounds__(64) void k_gauss_walk(const GaussStream *__restrict
                 first_done = true;
             }
         }
-        const uint32_t need = 4096u - (uint32_t)filled;
+        const uint32_t need = (uint32_t)S->slen - (uint32_t)filled;
         const uint64_t hit = __ballot(incl >= need);
         int e_chunk = -1;  // last chunk of this call, relative to c
         if (hit) {
@@ -501,20 +516,16 @@ __global__ void k_gauss_scatter(const GaussStream *__restrict__ streams, int nst
     if (info < 0) continue;
     const int n = cnt[gc];
     const int call = info >> 13, off = info & 0x1fff;
-    if (slot >= n || off + slot >= IPFE_N) continue;
+    if (slot >= n) continue;
     const int si = find_stream(chunk_prefix, nstreams, gc);
     const GaussStream *S = streams + si;
+    if (off + slot >= S->slen) continue;
     const uint64_t v = vals[gc * 8 + slot];
     const uint64_t z = v & 0x7fffffffffffffffull;
     const bool neg = (v >> 63) != 0;
     // output residues (src/gauss.c:773-779)
     uint32_t *out = ((call == 0) ? S->out0 : S->out1 + (long long)(call - 1) * S->out_stride) + off + slot;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        uint32_t m = mod_prime_exact(z, s);
-        if (neg) m = mod_prime_exact((uint32_t)(qmod(s) - m), s);
-        out[s * IPFE_N] = m;
-    }
+    for (int s = 0; s < S->nmod; ++s) out[(long long)s * S->slen] = stream_residue(S, s, z, neg);
     }
 }
 
@@ -524,23 +535,27 @@ __global__ void k_fill_i32(int *p, int v, size_t n)
 }
 
 // sample_polya (src/sample.c:56-128): coefficient i = LE64(keystream[8i..8i+8))
-// mod q_j, keystream of AES-256-CTR(seed, nonce 0) from counter 0.
+// mod q_j, keystream of AES-256-CTR(seed, nonce 0) from counter 0.  Any
+// parameter set: n coefficients on nmod limbs, a [nmod][n].
+struct PolyaArgs {
+    int n, nmod;
+    uint32_t q[4];
+};
 __global__ void k_sample_polya(const uint32_t *__restrict__ rk, const uint32_t *__restrict__ te_global,
-                               uint32_t *__restrict__ a)
+                               uint32_t *__restrict__ a, PolyaArgs pa)
 {
     __shared__ uint32_t te[AES_TABLE_WORDS];
     aes_stage_tables(te, te_global);
     __syncthreads();
-    const uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x;  // 2048 blocks
-    if (blk >= IPFE_N / 2) return;
+    const uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x;  // n / 2 blocks
+    if (blk >= (uint32_t)pa.n / 2) return;
     uint32_t o[4];
     aes256_ctr_block(rk, te, 0u, blk, o);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const uint64_t v = ((uint64_t)o[2 * h + 1] << 32) | o[2 * h];
         const uint32_t i = 2 * blk + h;
-#pragma unroll
-        for (int s = 0; s < 3; ++s) a[s * IPFE_N + i] = (uint32_t)(v % qmod(s));
+        for (int s = 0; s < pa.nmod; ++s) a[(size_t)s * pa.n + i] = (uint32_t)(v % pa.q[s]);
     }
 }
 
@@ -643,7 +658,21 @@ int launch_fill_i32(int *p, int v, size_t n, hipStream_t st)
 
 int launch_sample_polya(const uint32_t *d_rk, uint32_t *a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sample_polya, dim3(IPFE_N / 2 / 256), dim3(256), 0, st, d_rk, g_aes_te, a);
+    const uint32_t q[3] = {qmod(0), qmod(1), qmod(2)};
+    return launch_sample_polya_ps(d_rk, a, IPFE_N, 3, q, st);
+}
+
+int launch_sample_polya_ps(const uint32_t *d_rk, uint32_t *a, int n, int nmod, const uint32_t *q, hipStream_t st)
+{
+    if (n < 512 || (n & (n - 1)) || nmod < 1 || nmod > 4) {
+        ipfe_set_error("sample_polya: unsupported shape n=%d nmod=%d", n, nmod);
+        return -1;
+    }
+    PolyaArgs pa{};
+    pa.n = n;
+    pa.nmod = nmod;
+    for (int s = 0; s < nmod; ++s) pa.q[s] = q[s];
+    hipLaunchKernelGGL(k_sample_polya, dim3((unsigned)(n / 2 / 256)), dim3(256), 0, st, d_rk, g_aes_te, a, pa);
     HIP_OK(hipGetLastError());
     return 0;
 }

@@ -46,21 +46,26 @@ int launch_point_mul(const uint32_t *a, const uint32_t *b, uint32_t *out, size_t
 
 // ----- Sampler (k_sampler.hip) -----
 // One AES-256-CTR Gaussian stream cut into 384-byte chunks (24 AES blocks).
-// Chunk c of the stream uses counters 24c .. 24c+23.  Calls take 4096 accepted
+// Chunk c of the stream uses counters 24c .. 24c+23.  Calls take slen accepted
 // candidates each; call 0 writes to out0, call k >= 1 to out1 + (k-1)*out_stride.
 struct GaussStream {
     uint32_t rk[60];         // expanded AES-256 key
     uint32_t nonce;          // 16-bit nonce
-    int which;               // 1, 2, 3 (S1, S2, S3)
-    int ncalls;              // number of 4096-sample calls
+    int which;               // sampler width index into the device tables: 1, 2, 3 (S1, S2, S3 of the
+                             // reference set), 4, 5 (the S2 / S3 widths of the N = 2048 set)
+    int ncalls;              // number of calls
     int nchunks;             // speculative window length (chunks)
     long long chunk_base;    // index of this stream's chunk 0 in the chunk arrays
-    uint32_t *out0;          // [3][N] destination of call 0
+    uint32_t *out0;          // [nmod][slen] destination of call 0
     uint32_t *out1;          // destination of call 1
     long long out_stride;    // words between consecutive calls from call 1 on
-    int coef0_only;          // 1: write only sample 0 of each call, 3 words at out0 + 3*call
-    int pad;
+    int coef0_only;          // 1: write only sample 0 of each call, nmod words at out0 + nmod*call
+    int slen;                // samples per call (the parameter set's N)
     long long first_base;    // compact streams: index of call 0 in the per-call first-chunk array
+    uint32_t q[4];           // output moduli, limbs 0 .. nmod-1
+    uint32_t fold[4];        // mod_prime fold shifts per limb: a | b << 8 (q = 2^a - 2^b + 1)
+    int nmod;                // output limbs
+    int pad;
 };
 
 // Chunks per tile of the candidate pass (the host's tile_prefix counts tiles of
@@ -89,6 +94,8 @@ int launch_fill_i32(int *p, int v, size_t n, hipStream_t st);
 
 // Uniform polynomial a (src/sample.c:56-128) on device from the expanded key.
 int launch_sample_polya(const uint32_t *d_rk, uint32_t *a, hipStream_t st);
+// The same for any parameter set: n coefficients on nmod limbs of moduli q.
+int launch_sample_polya_ps(const uint32_t *d_rk, uint32_t *a, int n, int nmod, const uint32_t *q, hipStream_t st);
 // Raw keystream (tests): nblocks AES blocks from counter ctr0.
 int launch_aes_ctr(const uint32_t *d_rk, uint32_t nonce, uint32_t ctr0, uint32_t nblocks, uint8_t *out, hipStream_t st);
 

@@ -1,6 +1,8 @@
 // runtime.cpp -- device state, error reporting and the Gaussian-sampler driver.
 #include "runtime.h"
 
+#include "pset.h"
+
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -98,32 +100,59 @@ StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncal
     r.out1 = out1;
     r.out_stride = out_stride;
     r.coef0_only = coef0_only;
+    r.slen = IPFE_N;
+    r.nmod = 3;
+    for (int s = 0; s < 4; ++s) {
+        r.q[s] = s < 3 ? qmod(s) : 0u;
+        r.fold[s] = s < 3 ? (fold_a(s) | (fold_b(s) << 8)) : 0u;
+    }
     return r;
 }
 
-// Chunks per call of the three samplers, measured on the reference algorithm:
-// S1 accepts 68.1% of its candidates (751.6 chunks per 4096 samples, standard
-// deviation 6.6); the Bernoulli step of S2 / S3 is distorted for large k (the
-// int-to-double trick of src/gauss.c:368-369 is wrong above 2^52), so S2 accepts
-// 53.8% (951.8 chunks, sd 10.1) and S3 38.7% (1323.4 chunks, sd 16.2).  The
-// window takes the mean plus 1.5% plus 8 standard deviations; level m doubles
-// the margin m times (a short window is detected and retried, never wrong).
-long long gauss_window(int which, int ncalls, int m)
+StreamReq make_stream_ps(const PsetSpec &ps, const uint8_t key[32], uint16_t nonce, int which, int ncalls,
+                         uint32_t *out0, uint32_t *out1, long long out_stride)
 {
-    static const double mean[4] = {0.0, 751.6, 951.8, 1323.4};
-    static const double sd[4] = {0.0, 6.6, 10.1, 16.2};
-    const int w = (which >= 1 && which <= 3) ? which : 3;
-    const double n = (double)ncalls;
+    StreamReq r = make_stream(key, nonce, ps.width[which], ncalls, out0, out1, out_stride, 0);
+    r.slen = ps.N;
+    r.nmod = ps.nmod;
+    for (int s = 0; s < 4; ++s) {
+        r.q[s] = s < ps.nmod ? ps.q[s] : 0u;
+        r.fold[s] = s < ps.nmod ? (ps.fa[s] | (ps.fb[s] << 8)) : 0u;
+    }
+    return r;
+}
+
+// Chunks per 4096 samples of each sampler width, measured on the reference
+// algorithm: S1 accepts 68.1% of its candidates (751.6 chunks per 4096
+// samples, standard deviation 6.6); the Bernoulli step of S2 / S3 is distorted
+// for large k (the int-to-double trick of src/gauss.c:368-369 is wrong above
+// 2^52), so S2 accepts 53.8% (951.8 chunks, sd 10.1) and S3 38.7% (1323.4
+// chunks, sd 16.2).  The N = 2048 set's widths 262147 and 524287 stay below
+// that range and accept 68.0%, as S1 (752.8 chunks per 4096, sd 6.6-6.8).  A
+// call of slen samples takes slen / 4096 of that (sd scaled by its square
+// root).  The window takes the mean plus 1.5% plus 8 standard deviations;
+// level m doubles the margin m times (a short window is detected and retried,
+// never wrong).
+long long gauss_window(int which, int slen, int ncalls, int m)
+{
+    static const double mean[6] = {0.0, 751.6, 951.8, 1323.4, 752.8, 753.4};
+    static const double sd[6] = {0.0, 6.6, 10.1, 16.2, 6.6, 6.8};
+    const int w = (which >= 1 && which <= 5) ? which : 3;
+    const double n = (double)ncalls * (double)slen / 4096.0;
     double margin = 0.015 * mean[w] * n + 8.0 * sd[w] * sqrt(n) + 64.0;
     for (int i = 0; i < m; ++i) margin *= 2.0;
     return (long long)ceil(mean[w] * n + margin);
 }
 
-size_t gauss_bytes(int which, int ncalls, int coef0_only)
+long long gauss_window(int which, int ncalls, int m) { return gauss_window(which, IPFE_N, ncalls, m); }
+
+size_t gauss_bytes(int which, int slen, int ncalls, int coef0_only)
 {
-    const long long ch = gauss_window(which, ncalls, 0);
+    const long long ch = gauss_window(which, slen, ncalls, 0);
     return coef0_only ? (size_t)ch * 1 + (size_t)ncalls * 8 : (size_t)ch * (1 + 64 + 4);
 }
+
+size_t gauss_bytes(int which, int ncalls, int coef0_only) { return gauss_bytes(which, IPFE_N, ncalls, coef0_only); }
 
 int GaussPlan::build(const StreamReq *reqs, int count, int lvl, bool alloc_meta)
 {
@@ -148,14 +177,20 @@ int GaussPlan::build(const StreamReq *reqs, int count, int lvl, bool alloc_meta)
         g.nonce = rq.nonce;
         g.which = rq.which;
         g.ncalls = rq.ncalls;
-        g.nchunks = (int)gauss_window(rq.which, rq.ncalls, level);
+        g.nchunks = (int)gauss_window(rq.which, rq.slen, rq.ncalls, level);
         g.chunk_base = chunks;
         g.out0 = rq.out0;
         g.out1 = rq.out1;
         g.out_stride = rq.out_stride;
         g.coef0_only = rq.coef0_only;
-        g.pad = 0;
+        g.slen = rq.slen;
         g.first_base = nfirst;
+        for (int t = 0; t < 4; ++t) {
+            g.q[t] = rq.q[t];
+            g.fold[t] = rq.fold[t];
+        }
+        g.nmod = rq.nmod;
+        g.pad = 0;
         tile_prefix[i] = tiles;
         chunk_prefix[i] = chunks;
         first_prefix[i] = nfirst;
@@ -303,8 +338,8 @@ int run_gauss(const std::vector<StreamReq> &reqs, hipStream_t st, size_t scratch
         size_t j = i, bytes = 0;
         long long nch = 0;
         while (j < reqs.size()) {
-            size_t b = gauss_bytes(reqs[j].which, reqs[j].ncalls, reqs[j].coef0_only);
-            long long c = gauss_window(reqs[j].which, reqs[j].ncalls, 0);
+            size_t b = gauss_bytes(reqs[j].which, reqs[j].slen, reqs[j].ncalls, reqs[j].coef0_only);
+            long long c = gauss_window(reqs[j].which, reqs[j].slen, reqs[j].ncalls, 0);
             if (j > i && (bytes + b > scratch_budget || nch + c > (1ll << 28))) break;
             bytes += b;
             nch += c;

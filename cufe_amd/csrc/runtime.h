@@ -62,15 +62,24 @@ struct DevBuf {
 struct StreamReq {
     uint32_t rk[60];
     uint16_t nonce;
-    int which;
+    int which;  // sampler width index (GaussStream::which)
     int ncalls;
     uint32_t *out0;
     uint32_t *out1;
     long long out_stride;
     int coef0_only;
+    int slen;          // samples per call
+    int nmod;          // output limbs and their moduli / fold shifts
+    uint32_t q[4], fold[4];
 };
+// A stream of the reference parameter set (4096 samples per call, 3 limbs);
+// which = 1, 2, 3 for S1, S2, S3.
 StreamReq make_stream(const uint8_t key[32], uint16_t nonce, int which, int ncalls, uint32_t *out0, uint32_t *out1,
                       long long out_stride, int coef0_only = 0);
+struct PsetSpec;
+// A stream of parameter set ps: sampler which = 1, 2, 3 at that set's widths.
+StreamReq make_stream_ps(const PsetSpec &ps, const uint8_t key[32], uint16_t nonce, int which, int ncalls,
+                         uint32_t *out0, uint32_t *out1, long long out_stride);
 
 // Samples every request (device outputs), bit-exact with the reference
 // sequential sampler; splits into groups that fit scratch_budget bytes.
@@ -121,9 +130,12 @@ struct GaussPlan {
     int check(hipStream_t st, int *overflow, int *need_full);
 };
 
-// Speculative window (chunks) for a stream of ncalls calls of sampler `which`
-// at margin level m, and the scratch bytes one stream needs at level 0.
+// Speculative window (chunks) for a stream of ncalls calls of slen samples of
+// sampler width `which` at margin level m, and the scratch bytes one stream
+// needs at level 0 (the three-argument forms: 4096 samples per call).
+long long gauss_window(int which, int slen, int ncalls, int m);
 long long gauss_window(int which, int ncalls, int m);
+size_t gauss_bytes(int which, int slen, int ncalls, int coef0_only);
 size_t gauss_bytes(int which, int ncalls, int coef0_only);
 
 int sync_stream(hipStream_t st);

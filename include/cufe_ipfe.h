@@ -162,6 +162,61 @@ int cufe_time_gauss(int nstreams, int which, int iters, float *ms_per_launch);
 int cufe_time_gauss_calls(int nstreams, int ncalls, int which, int iters, float *ms_per_run);
 
 /* ===================================================================== */
+/* 3b. Parameter sets                                                      */
+/*    pset 0: N = 2048, 2 limbs (2^31-2^17+1, 2^31-2^24+1), sampler widths  */
+/*            k = 266, 262147, 524287;                                      */
+/*    pset 1: the reference's set (src/params.h:14-28), the set of every    */
+/*            other function in this header;                                */
+/*    pset 2: N = 8192, 4 limbs (the reference's primes and 2^26-2^16+1),   */
+/*            the reference's sampler widths.                               */
+/*    L = 785, p = 50241 in each.  The reference defines only set 1; the    */
+/*    others follow its comment on other levels (src/sample.c:16-19).       */
+/*    Layouts as above with nmod limbs and N coefficients:                  */
+/*    poly[slot][limb][coefficient].  Same seed convention as section 1.    */
+/* ===================================================================== */
+
+/* N, limbs, key length, moduli q[4] (unused limbs 0) and p of a set. */
+int cufe_pset_info(int pset, int *n, int *nmod, int *l, uint32_t *q, uint32_t *p);
+/* CT_forward / GS_reverse (src/ntt.c:38-114) of every limb of polys
+ * [npoly][nmod][N], in place, with the shape the autotuner picked for the set
+ * and direction on this GPU (tuned on first use). */
+int cufe_ps_ntt(int pset, int inverse, uint32_t *polys, int npoly);
+/* The same with one given shape (all shapes give identical results). */
+int cufe_ps_ntt_variant(int pset, int variant, int inverse, uint32_t *polys, int npoly);
+/* Number of NTT shapes for the set's N; names (optional) receives one name per
+ * line ("-" for a shape the set cannot use). */
+int cufe_ntt_variants(int pset, char *names, int len);
+/* The NTT autotuner's timings and choices so far.  Returns the length. */
+int cufe_ntt_autotune_report(char *buf, int len);
+/* sample_polya (src/sample.c:56-128) at the set's N: a [nmod][N]. */
+int cufe_ps_sample_polya(int pset, const unsigned char seed[32], uint32_t *a);
+/* gaussian_sampler_S{which} with the set's widths and N samples per call:
+ * out [nstreams][ncalls][nmod][N]. */
+int cufe_ps_gauss_sample(int pset, const unsigned char *keys, const uint16_t *nonces, int nstreams, int which,
+                         int ncalls, uint32_t *out);
+/* rlwe_sife_setup / encrypt / keygen / decrypt_gmp at the set's parameters,
+ * with the layouts of cufe_sife_* (mpk [l+1][nmod][N] with mpk[l] = a on entry
+ * of setup and a_hat on return, msk [l][nmod][N], c [repeat][l+1][nmod][N],
+ * sk / d [repeat][nmod][N]). */
+int cufe_ps_setup(int pset, uint32_t *mpk, uint32_t *msk, const unsigned char seed2[32],
+                  const unsigned char seed3[32], int l);
+int cufe_ps_encrypt(int pset, const uint32_t *m, const uint32_t *mpk, uint32_t *c, const unsigned char seed2[32],
+                    const unsigned char seed3[32], int repeat, int l, int nonce0);
+int cufe_ps_keygen(int pset, const uint32_t *y, const uint32_t *msk, uint32_t *sk, int repeat, int l);
+int cufe_ps_decrypt(int pset, const uint32_t *c, const uint32_t *y, const uint32_t *sk, uint32_t *d, int repeat,
+                    int l);
+/* crt_reverse_gmp + round_extract_gmp (src/crt.c:46-82, src/rlwe_sife.c:288-332)
+ * on the GPU: coefficient coef of d [nvec][nmod][N] -> r [nvec], or every
+ * coefficient when coef < 0 -> r [nvec][N]. */
+int cufe_ps_crt_round(int pset, const uint32_t *d, int nvec, int coef, uint32_t *r);
+/* The same for one coefficient's residues res [nmod], on the host. */
+int cufe_ps_crt_round_host(int pset, const uint32_t *res, uint32_t *r);
+/* Device-resident timing: the tuned transform over npoly polys of the set
+ * (ms per launch), and nstreams one-call Gaussian streams (ms per run). */
+int cufe_time_ps_ntt(int pset, int npoly, int inverse, int iters, float *ms_per_launch);
+int cufe_time_ps_gauss(int pset, int nstreams, int which, int iters, float *ms_per_run);
+
+/* ===================================================================== */
 /* 4. Resident-key context and batched prediction                          */
 /*    Keys, functional keys and encrypted rows stay in HBM; one predict     */
 /*    call classifies many rows against every support vector at once.       */

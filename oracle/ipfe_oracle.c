@@ -405,7 +405,12 @@ static int bernoulli(uint64_t b, uint64_t k, const uint8_t *rb, int lane)
  *   base+64 .. base+99; uniform words from byte 200; sign bits in byte 280. */
 int orc_gauss_chunk(const uint8_t r[384], int which, uint64_t z[8], uint8_t sign[8], uint8_t acc[8])
 {
-    const uint64_t k = SAMPLER_K[which];
+    return orc_gauss_chunk_k(r, SAMPLER_K[which], z, sign, acc);
+}
+
+/* The chunk step for any sampler width k (S1 / S2 / S3 differ only in k). */
+int orc_gauss_chunk_k(const uint8_t r[384], uint64_t k, uint64_t z[8], uint8_t sign[8], uint8_t acc[8])
+{
     const uint64_t bf = (1ull << 32) / k;
     const uint64_t uq = k * bf;
     uint64_t y[8];

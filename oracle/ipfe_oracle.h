@@ -65,6 +65,8 @@ void orc_gauss(orc_stream *st, int which, uint32_t out[ORC_NMODULI][ORC_N], uint
  * and for the GPU candidate-kernel parity check.  Returns 0, or -1 when the
  * uniform sampler would read past the chunk. */
 int orc_gauss_chunk(const uint8_t r[384], int which, uint64_t z[8], uint8_t sign[8], uint8_t acc[8]);
+/* The same for any sampler width k (S1/S2/S3 use k = 266, 304214978, 608429953). */
+int orc_gauss_chunk_k(const uint8_t r[384], uint64_t k, uint64_t z[8], uint8_t sign[8], uint8_t acc[8]);
 
 /* src/ntt.c:38-67, :70-114, :28-36 */
 void orc_ct_forward(uint32_t a[ORC_N], uint32_t sel);
@@ -112,6 +114,37 @@ double orc_svm_decide(int nr_class, int l_sv, const int *nSV, const int *label,
 double orc_svm_predict_cpu(const uint32_t *c, int l, const uint32_t *msk, int nr_class, int l_sv,
                            const int *nSV, const int *label, const double *sv_coef,
                            const double *rho, const uint32_t *y, double *kvalue_out, int use_fma);
+
+/* ------------------------------------------------------------------------- */
+/* The path for any of the library's three parameter sets (pset_oracle.c):     */
+/* set 0: N = 2048, 2 limbs; set 1: the reference's set; set 2: N = 8192, 4    */
+/* limbs.  Layouts are those above with nmod limbs and N coefficients.         */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int id, N, logN, nmod, L;
+    uint32_t q[4], fa[4], fb[4], root[4]; /* q = 2^fa - 2^fb + 1; root: primitive 2N-th */
+    uint64_t k[4];                        /* sampler widths k[1..3] */
+    uint32_t p;
+} orc_pset;
+
+int orc_pset_get(int id, orc_pset *out);
+uint32_t orc_ps_mod_prime(const orc_pset *ps, uint64_t m, int j);
+void orc_ps_psi(const orc_pset *ps, int j, uint32_t *psi, uint32_t *psi_inv);
+uint32_t orc_ps_ninv(const orc_pset *ps, int j);
+uint32_t orc_ps_scale(const orc_pset *ps, int j);
+void orc_ps_ct_forward(const orc_pset *ps, uint32_t *a, int j);
+void orc_ps_gs_reverse(const orc_pset *ps, uint32_t *a, int j);
+void orc_ps_sample_polya(const orc_pset *ps, const uint8_t seed[32], uint32_t *a);
+void orc_ps_gauss(const orc_pset *ps, orc_stream *st, int which, uint32_t *out);
+void orc_ps_setup_from_a(const orc_pset *ps, const uint32_t *a_coeff, const uint8_t seedS[32],
+                         const uint8_t seedE[32], int l, uint32_t *mpk, uint32_t *msk);
+void orc_ps_encrypt(const orc_pset *ps, const uint32_t *x, int l, const uint32_t *mpk, const uint8_t seed2[32],
+                    const uint8_t seed3[32], uint16_t nonce, uint32_t *c);
+void orc_ps_keygen(const orc_pset *ps, const uint32_t *y, int l, const uint32_t *msk, uint32_t *sk);
+void orc_ps_decrypt(const orc_pset *ps, const uint32_t *c, const uint32_t *y, int l, const uint32_t *sk,
+                    uint32_t *d);
+void orc_ps_crt(const orc_pset *ps, const uint32_t *res, uint64_t *lo, uint64_t *hi);
+uint32_t orc_ps_round(const orc_pset *ps, uint64_t lo, uint64_t hi);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,26 @@ def orc() -> C.CDLL:
         o.orc_svm_decide.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int]
         o.orc_kvalues_coef0.restype = C.c_int
         o.orc_kvalues_coef0.argtypes = [_u32p, _u32p, C.c_int, C.c_int, _u32p, C.c_int, _u32p, C.c_int, _u32p]
+        psp = C.POINTER(PSet)
+        o.orc_pset_get.argtypes = [C.c_int, psp]
+        o.orc_ps_mod_prime.restype = C.c_uint32
+        o.orc_ps_mod_prime.argtypes = [psp, C.c_uint64, C.c_int]
+        o.orc_ps_psi.argtypes = [psp, C.c_int, _u32p, _u32p]
+        o.orc_ps_ninv.restype = C.c_uint32
+        o.orc_ps_ninv.argtypes = [psp, C.c_int]
+        o.orc_ps_scale.restype = C.c_uint32
+        o.orc_ps_scale.argtypes = [psp, C.c_int]
+        o.orc_ps_ct_forward.argtypes = [psp, _u32p, C.c_int]
+        o.orc_ps_gs_reverse.argtypes = [psp, _u32p, C.c_int]
+        o.orc_ps_sample_polya.argtypes = [psp, _u8p, _u32p]
+        o.orc_ps_gauss.argtypes = [psp, C.POINTER(Stream), C.c_int, _u32p]
+        o.orc_ps_setup_from_a.argtypes = [psp, _u32p, _u8p, _u8p, C.c_int, _u32p, _u32p]
+        o.orc_ps_encrypt.argtypes = [psp, _u32p, C.c_int, _u32p, _u8p, _u8p, C.c_uint16, _u32p]
+        o.orc_ps_keygen.argtypes = [psp, _u32p, C.c_int, _u32p, _u32p]
+        o.orc_ps_decrypt.argtypes = [psp, _u32p, _u32p, C.c_int, _u32p, _u32p]
+        o.orc_ps_crt.argtypes = [psp, _u32p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        o.orc_ps_round.restype = C.c_uint32
+        o.orc_ps_round.argtypes = [psp, C.c_uint64, C.c_uint64]
         o.orc_svm_predict_cpu.restype = C.c_double
         o.orc_svm_predict_cpu.argtypes = [_u32p, C.c_int, _u32p, C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _u32p,
                                           _f64p, C.c_int]
@@ -237,6 +257,140 @@ def crt_value(d0: int, d1: int, d2: int) -> int:
     lo, hi = C.c_uint64(), C.c_uint64()
     orc().orc_crt_reverse(d0, d1, d2, C.byref(lo), C.byref(hi))
     return (hi.value << 64) | lo.value
+
+
+# ---- parameter sets (pset_oracle.c) ---------------------------------------------
+class PSet(C.Structure):
+    _fields_ = [("id", C.c_int), ("N", C.c_int), ("logN", C.c_int), ("nmod", C.c_int), ("L", C.c_int),
+                ("q", C.c_uint32 * 4), ("fa", C.c_uint32 * 4), ("fb", C.c_uint32 * 4), ("root", C.c_uint32 * 4),
+                ("k", C.c_uint64 * 4), ("p", C.c_uint32)]
+
+    @property
+    def qs(self):
+        return [int(self.q[j]) for j in range(self.nmod)]
+
+    @property
+    def Q(self):
+        out = 1
+        for q in self.qs:
+            out *= q
+        return out
+
+
+_ps_cache = {}
+
+
+def pset(pid: int) -> PSet:
+    if pid not in _ps_cache:
+        ps = PSet()
+        if orc().orc_pset_get(pid, C.byref(ps)) != 0:
+            raise ValueError(f"no parameter set {pid}")
+        _ps_cache[pid] = ps
+    return _ps_cache[pid]
+
+
+def _psp(pid):
+    return C.byref(pset(pid))
+
+
+def ps_mod_prime(pid: int, m: int, j: int) -> int:
+    return int(orc().orc_ps_mod_prime(_psp(pid), m, j))
+
+
+def ps_psi(pid: int, j: int):
+    ps = pset(pid)
+    a, b = np.empty(ps.N, np.uint32), np.empty(ps.N, np.uint32)
+    orc().orc_ps_psi(_psp(pid), j, a, b)
+    return a, b
+
+
+def ps_ntt_forward(pid: int, polys) -> np.ndarray:
+    """[..][nmod][N] -> CT_forward of every limb."""
+    ps = pset(pid)
+    a = np.array(polys, dtype=np.uint32, copy=True).reshape(-1, ps.nmod, ps.N)
+    for p in range(a.shape[0]):
+        for j in range(ps.nmod):
+            row = np.ascontiguousarray(a[p, j])
+            orc().orc_ps_ct_forward(_psp(pid), row, j)
+            a[p, j] = row
+    return a.reshape(np.shape(polys))
+
+
+def ps_ntt_inverse(pid: int, polys) -> np.ndarray:
+    ps = pset(pid)
+    a = np.array(polys, dtype=np.uint32, copy=True).reshape(-1, ps.nmod, ps.N)
+    for p in range(a.shape[0]):
+        for j in range(ps.nmod):
+            row = np.ascontiguousarray(a[p, j])
+            orc().orc_ps_gs_reverse(_psp(pid), row, j)
+            a[p, j] = row
+    return a.reshape(np.shape(polys))
+
+
+def ps_sample_polya(pid: int, seed) -> np.ndarray:
+    ps = pset(pid)
+    a = np.empty((ps.nmod, ps.N), np.uint32)
+    orc().orc_ps_sample_polya(_psp(pid), seed_bytes(seed), a)
+    return a
+
+
+def ps_gauss_calls(pid: int, key, nonce: int, which: int, ncalls: int) -> np.ndarray:
+    ps = pset(pid)
+    st = stream(key, nonce)
+    out = np.empty((ncalls, ps.nmod, ps.N), np.uint32)
+    for c in range(ncalls):
+        buf = np.empty((ps.nmod, ps.N), np.uint32)
+        orc().orc_ps_gauss(_psp(pid), C.byref(st), which, buf)
+        out[c] = buf
+    return out
+
+
+def ps_setup_from_a(pid: int, a, seedS, seedE, l: int):
+    ps = pset(pid)
+    mpk = np.zeros((l + 1, ps.nmod, ps.N), np.uint32)
+    msk = np.zeros((max(l, 1), ps.nmod, ps.N), np.uint32)
+    orc().orc_ps_setup_from_a(_psp(pid), np.ascontiguousarray(a, np.uint32), seed_bytes(seedS), seed_bytes(seedE),
+                              l, mpk, msk)
+    return mpk, msk[:l]
+
+
+def ps_encrypt(pid: int, x, mpk, seed2, seed3, nonce: int) -> np.ndarray:
+    ps = pset(pid)
+    x = np.ascontiguousarray(x, np.uint32)
+    l = x.size
+    c = np.empty((l + 1, ps.nmod, ps.N), np.uint32)
+    orc().orc_ps_encrypt(_psp(pid), x if l else np.zeros(1, np.uint32), l, np.ascontiguousarray(mpk, np.uint32),
+                         seed_bytes(seed2), seed_bytes(seed3), nonce, c)
+    return c
+
+
+def ps_keygen(pid: int, y, msk) -> np.ndarray:
+    ps = pset(pid)
+    y = np.ascontiguousarray(y, np.uint32)
+    sk = np.empty((ps.nmod, ps.N), np.uint32)
+    orc().orc_ps_keygen(_psp(pid), y, y.size, np.ascontiguousarray(msk, np.uint32), sk)
+    return sk
+
+
+def ps_decrypt(pid: int, c, y, sk) -> np.ndarray:
+    ps = pset(pid)
+    y = np.ascontiguousarray(y, np.uint32)
+    d = np.empty((ps.nmod, ps.N), np.uint32)
+    orc().orc_ps_decrypt(_psp(pid), np.ascontiguousarray(c, np.uint32), y, y.size, np.ascontiguousarray(sk, np.uint32),
+                         d)
+    return d
+
+
+def ps_crt_value(pid: int, res) -> int:
+    lo, hi = C.c_uint64(), C.c_uint64()
+    orc().orc_ps_crt(_psp(pid), np.ascontiguousarray(res, np.uint32), C.byref(lo), C.byref(hi))
+    return (hi.value << 64) | lo.value
+
+
+def ps_crt_round(pid: int, res) -> int:
+    lo, hi = C.c_uint64(), C.c_uint64()
+    orc().orc_ps_crt(_psp(pid), np.ascontiguousarray(res, np.uint32), C.byref(lo), C.byref(hi))
+    return int(orc().orc_ps_round(_psp(pid), lo.value, hi.value))
 
 
 # ---- reference-library conveniences ---------------------------------------------

@@ -43,3 +43,16 @@ def gpu():
 
     cufe_amd.lib()  # raises if the HIP library is missing: no fallback
     return cufe_amd
+
+
+@pytest.fixture(scope="session")
+def gpu_lib_optional():
+    """The HIP library for host-only functions (CRT on the host), or None when
+    it is not built; nothing here touches the device."""
+    try:
+        import cufe_amd
+
+        cufe_amd.lib()
+        return cufe_amd
+    except Exception:
+        return None

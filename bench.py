@@ -319,6 +319,47 @@ def single_ops(cf, ctx, rng):
                                           "decrypt": 1e3 * (t3 - t2),
                                           "decrypted_equals_inner_product_mod_p":
                                               int(kv) % P_MOD == int(x[0].astype(np.int64) @ y[0].astype(np.int64))}
+    out["parameter_sets"] = {str(pid): pset_single_ops(cf, pid, rng) for pid in (0, 1, 2)}
+    out["ntt_autotune"] = cf.ntt_autotune_report().strip().split("\n")
+    return out
+
+
+def pset_single_ops(cf, pid, rng):
+    """The single-operation kernels at one parameter set (0: N = 2048 / 2 limbs,
+    1: the reference set, 2: N = 8192 / 4 limbs): forward and inverse NTT of
+    10,000 polynomials with the autotuned shape (HBM: 8 N bytes per limb
+    transform), 10,000 one-call Gaussian streams per sampler, and one setup /
+    encrypt / keygen / decrypt + CRT at l = 123 on host buffers, the decryption
+    checked against <x, y> on every coefficient."""
+    ps = cf.PSet(pid)
+    out = {"N": ps.N, "limbs": ps.nmod}
+    for name, inv in (("ntt_forward", False), ("ntt_inverse", True)):
+        ms = ps.time_ntt(10000, inv, 10)
+        limbs = 10000 * ps.nmod / (ms * 1e-3)
+        out[name] = {"limb_transforms_per_s": limbs, "ms_per_10000_polys": ms,
+                     "hbm_gb_s": limbs * 8 * ps.N / 1e9, "frac_of_hbm": limbs * 8 * ps.N / 8e12}
+    for which in (1, 2, 3):
+        ms = ps.time_gauss(10000, which, 2)
+        out[f"gauss_S{which}_10000_streams"] = {"ms": ms, "polys_per_s": 10000 / (ms * 1e-3)}
+    l = 123
+    a = ps.sample_polya(h(b"pso-a%d" % pid))
+    t0 = time.perf_counter()
+    mpk, msk = ps.setup(a, h(b"pso-s"), h(b"pso-e"), l)
+    t1 = time.perf_counter()
+    x = rng.integers(0, 5, (1, l)).astype(np.uint32)
+    y = rng.integers(0, 17, (1, l)).astype(np.uint32)
+    c = ps.encrypt(x, mpk, h(b"pso-2"), h(b"pso-3"))
+    t2 = time.perf_counter()
+    sk = ps.keygen(y, msk)
+    t3 = time.perf_counter()
+    d = ps.decrypt(c[0], y, sk)
+    r = ps.crt_round(d, coef=-1)
+    t4 = time.perf_counter()
+    ip = int(x[0].astype(np.int64) @ y[0].astype(np.int64))
+    out["scheme_l123_host_buffers_ms"] = {"setup": 1e3 * (t1 - t0), "encrypt": 1e3 * (t2 - t1),
+                                          "keygen": 1e3 * (t3 - t2), "decrypt_and_crt": 1e3 * (t4 - t3),
+                                          "every_coefficient_decrypts_to_inner_product":
+                                              bool(np.all(r.astype(np.int64) % ps.p == ip % ps.p))}
     return out
 
 
@@ -621,6 +662,11 @@ def compact_line(out: dict, detail: str | None = None) -> str:
                                     "limb_transforms_per_s" in v else v.get("polys_per_s"))
                               for k, v in so.items() if isinstance(v, dict) and
                               ("limb_transforms_per_s" in v or "polys_per_s" in v)}
+        for pid, v in so.get("parameter_sets", {}).items():
+            line["single_ops"][f"set{pid}_N{v['N']}_ntt_fwd_inv"] = [_r(v["ntt_forward"]["limb_transforms_per_s"]),
+                                                                    _r(v["ntt_inverse"]["limb_transforms_per_s"])]
+            line["single_ops"][f"set{pid}_decrypts_ok"] = \
+                v["scheme_l123_host_buffers_ms"]["every_coefficient_decrypts_to_inner_product"]
     if detail:
         line["detail"] = detail
     text = json.dumps(line, separators=(",", ":"))

@@ -46,7 +46,13 @@ def representative_result():
                        "ntt_forward": {"limb_transforms_per_s": 8.9e7, "ms_per_10000_polys": 0.33},
                        "ntt_inverse": {"limb_transforms_per_s": 8.8e7, "ms_per_10000_polys": 0.34},
                        "gauss_S1_10000_streams": {"ms": 6.1, "polys_per_s": 1.6e6},
-                       "gauss_S3_10000_streams": {"ms": 11.0, "polys_per_s": 9.1e5}},
+                       "gauss_S3_10000_streams": {"ms": 11.0, "polys_per_s": 9.1e5},
+                       "parameter_sets": {str(pid): {"N": 2048 << pid, "limbs": 2 + pid,
+                                                     "ntt_forward": {"limb_transforms_per_s": 9.1e7},
+                                                     "ntt_inverse": {"limb_transforms_per_s": 9.0e7},
+                                                     "scheme_l123_host_buffers_ms": {
+                                                         "every_coefficient_decrypts_to_inner_product": True}}
+                                          for pid in range(3)}},
         "batched_encrypt": {"messages": 4096, "eager_ms": 1332.5, "graph_ms": 1331.6,
                             "encryptions_per_s_graph": 3076.0, "batched_vs_plain": 49.0,
                             "graph_equals_eager": True, "message0_equals_plain_abi_prefix": True,
@@ -78,6 +84,7 @@ def test_line_is_compact_and_complete():
         assert k in line["cpu_baseline"], k
     assert "autotune" not in line
     assert line["detail"] == "gpurun_out/bench_detail.json"
+    assert line["single_ops"]["set2_N8192_ntt_fwd_inv"] == [9.1e7, 9.0e7] and line["single_ops"]["set0_decrypts_ok"]
 
 
 def test_line_sheds_optional_blocks_when_too_long():

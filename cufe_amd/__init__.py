@@ -736,6 +736,8 @@ def _server_sigs(lb):
                                          C.POINTER(C.c_double)]),
         "cufe_server_stats": (C.c_int, [p, C.POINTER(C.c_longlong)]),
         "cufe_server_reset_stats": (C.c_int, [p]),
+        "cufe_server_info": (C.c_int, [p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                       C.POINTER(C.c_longlong)]),
         "cufe_server_destroy": (None, [p]),
     }
     for name, (res, args) in sig.items():
@@ -824,6 +826,15 @@ class Server:
         _check(lib().cufe_server_stats(self._h, out), "server_stats")
         keys = ("requests", "rows", "batches", "failed", "max_batch_rows", "cap_rows", "model_bytes", "slot_bytes")
         return dict(zip(keys, list(out)))
+
+    def info(self) -> dict:
+        """Slots, requests split into parts, peak HBM bytes of the batches in
+        flight, and the device bytes one batch row occupies."""
+        ns = C.c_int()
+        sp, pk, br = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        _check(lib().cufe_server_info(self._h, C.byref(ns), C.byref(sp), C.byref(pk), C.byref(br)), "server_info")
+        return {"slots": ns.value, "split_requests": sp.value, "peak_inflight_bytes": pk.value,
+                "bytes_per_row": br.value}
 
     def reset_stats(self):
         """Zero the request, row, batch, failure and largest-batch counters."""

@@ -5,14 +5,18 @@
 // every row and regenerates every functional key.  Here the keys of every
 // loaded model stay in HBM; requests carry rows in compact ciphertext form
 // (c_L and coefficient 0 of c_0..c_{l-1}), and a worker thread packs whatever
-// requests are queued into the next batch, as many as the batch region holds.
+// requests are queued into the next batch, as many as a batch slot holds.
 //
-// Device memory is fixed at start: the models' keys plus two batch slots whose
-// row capacity is what the HBM budget leaves.  While one slot's batch runs on
-// its stream, the next batch is staged and launched on the other slot's
-// stream.  A row's result depends only on the row and its model (the
-// contraction of a row never mixes other rows), so a request returns the same
-// bits whether it runs alone or among others.
+// Device memory is fixed at start: the models' keys plus 2 to 4 batch slots,
+// each of a row capacity the HBM budget affords (every row of a batch holds
+// its row store, staging, contraction scratch and key values, so a request of
+// n rows occupies n times that many bytes while it is in flight).  A request
+// larger than a slot is split into slot-sized parts that run as separate
+// batches (possibly side by side) and complete the request together.  While
+// one slot's batch runs on its stream, the next is staged and launched on
+// another slot's stream.  A row's result depends only on the row and its
+// model (the contraction of a row never mixes other rows), so a request
+// returns the same bits whether it runs alone, among others or in parts.
 #include <stdio.h>
 #include <string.h>
 
@@ -38,17 +42,26 @@ constexpr size_t kPoly = (size_t)3 * IPFE_N;
 
 double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
 
+// A submitted request: done when all of its parts are.
+struct Ticket {
+    uint64_t id = 0;
+    double t_arrival = 0, t_done = 0;
+    int parts_left = 0;
+    int status = 0;  // 0 queued or running, 1 done, -1 failed
+    std::string err;
+};
+using TicketP = std::shared_ptr<Ticket>;
+
+// One part of a request (the whole request unless it exceeds a slot): n rows
+// of the caller's buffers, offset to the part.
 struct Req {
-    uint64_t ticket = 0;
+    TicketP tk;
     int model = 0;
     const Model *m = nullptr;
     int n = 0, l = 0;
     const uint32_t *cl = nullptr, *c0 = nullptr;
     double *labels = nullptr;
     uint32_t *kvalues = nullptr;
-    double t_arrival = 0, t_done = 0;
-    int status = 0;  // 0 queued or running, 1 done, -1 failed
-    std::string err;
 };
 using ReqP = std::shared_ptr<Req>;
 
@@ -75,6 +88,7 @@ struct Slot {
     std::vector<Group> groups;
     bool busy = false;
     int batch_rows = 0;  // rows of the batch in flight
+    size_t batch_bytes = 0;  // HBM its rows occupy (rows x bytes per row)
     bool failed = false;
     std::string err;
 };
@@ -92,16 +106,18 @@ struct cufe_server {
     std::vector<std::unique_ptr<Model>> models;
     size_t model_bytes = 0;
     int S_max = 0, l_max = 0, cap = 0;
-    size_t slot_bytes = 0;
-    Slot slots[2];
+    size_t slot_bytes = 0, row_bytes = 0;
+    std::vector<std::unique_ptr<Slot>> slots;  // 2 to 4, sized at start
     std::mutex mu;
     std::condition_variable cv_submit, cv_done;
     std::deque<ReqP> pending;
-    std::unordered_map<uint64_t, ReqP> live;
+    std::unordered_map<uint64_t, TicketP> live;
     uint64_t next_ticket = 1;
     std::thread worker;
     bool started = false, stop = false;
     long long st_requests = 0, st_rows = 0, st_batches = 0, st_failed = 0, st_batch_rows_max = 0;
+    long long st_split = 0;           // requests split into parts
+    size_t inflight_bytes = 0, st_inflight_peak = 0;  // HBM held by batches in flight
 };
 
 namespace ipfe {
@@ -112,6 +128,10 @@ int ctx_key_len(const cufe_ctx *c);
 namespace {
 
 inline int round32(int x) { return (x + 31) & ~31; }
+
+// Rows a batch gathers from the queue: the largest batch whose contraction
+// still runs as the single split-K launch (predict.cpp, kSplitMaxRows).
+constexpr long long kBatchRows = 256;
 
 // Stages the batch in sl (host copies, H2D, row-store puts) and launches its
 // prediction; returns -1 with the error set if a call fails.
@@ -190,12 +210,15 @@ int launch_batch(cufe_server *s, Slot &sl)
             return -1;
         i = j;
     }
-    // small groups take the low-latency split-K launch unless the other slot
+    // small groups take the low-latency split-K launch unless another slot
     // runs a large batch (under load the split launch would slow it more than
     // it gains)
-    const Slot &other = (&sl == &s->slots[0]) ? s->slots[1] : s->slots[0];
     sl.batch_rows = row;
-    sl.sc.split_ok = !other.busy || other.batch_rows <= 256;
+    sl.batch_bytes = (size_t)row * s->row_bytes;
+    bool large_other = false;
+    for (const auto &o : s->slots)
+        if (o.get() != &sl && o->busy && o->batch_rows > 256) large_other = true;
+    sl.sc.split_ok = !large_other;
     for (size_t g = 0; g < sl.groups.size(); ++g) {
         const Group &gr = sl.groups[g];
         if (predict_rows(*gr.m, sl.rows, gr.r0, gr.R, sl.sc, s->cap, mode, sl.h_labels + gr.r0,
@@ -210,8 +233,8 @@ int launch_batch(cufe_server *s, Slot &sl)
     return 0;
 }
 
-// Copies results of a finished slot to the callers' buffers and completes its
-// requests.
+// Copies results of a finished slot to the callers' buffers and completes the
+// parts it ran (a request completes with its last part).
 void retire(cufe_server *s, Slot &sl, hipError_t q)
 {
     if (q != hipSuccess && !sl.failed) {
@@ -238,20 +261,23 @@ void retire(cufe_server *s, Slot &sl, hipError_t q)
         int rows = 0;
         for (size_t i = 0; i < sl.reqs.size(); ++i) {
             Req &r = *sl.reqs[i];
-            r.t_done = t;
+            Ticket &tk = *r.tk;
             if (sl.failed || !errs[i].empty()) {
-                r.status = -1;
-                r.err = sl.failed ? sl.err : errs[i];
-                s->st_failed++;
-            } else {
-                r.status = 1;
+                if (tk.err.empty()) tk.err = sl.failed ? sl.err : errs[i];
             }
             rows += r.n;
+            if (--tk.parts_left == 0) {
+                tk.t_done = t;
+                tk.status = tk.err.empty() ? 1 : -1;
+                s->st_requests++;
+                if (tk.status < 0) s->st_failed++;
+            }
         }
-        s->st_requests += (long long)sl.reqs.size();
         s->st_rows += rows;
         s->st_batches++;
         s->st_batch_rows_max = std::max<long long>(s->st_batch_rows_max, rows);
+        s->inflight_bytes -= sl.batch_bytes;
+        sl.batch_bytes = 0;
         sl.reqs.clear();
         sl.busy = false;
         sl.failed = false;
@@ -265,33 +291,30 @@ void worker_main(cufe_server *s)
     (void)hipSetDevice(0);
     for (;;) {
         bool progressed = false;
-        for (Slot &sl : s->slots) {
+        for (auto &slp : s->slots) {
+            Slot &sl = *slp;
             if (!sl.busy) continue;
             const hipError_t q = hipEventQuery(sl.done);
             if (q == hipErrorNotReady) continue;
             retire(s, sl, q);
             progressed = true;
         }
-        Slot *free_slot = nullptr;
-        for (Slot &sl : s->slots)
-            if (!sl.busy) {
-                free_slot = &sl;
-                break;
-            }
-        if (free_slot) {
-            Slot &sl = *free_slot;
+        for (auto &slp : s->slots) {
+            Slot &sl = *slp;
+            if (sl.busy) continue;
             {
                 std::lock_guard<std::mutex> lk(s->mu);
-                // admit queued requests in arrival order while the aligned
-                // layout fits the slot; when the other slot is free too, this
-                // one takes the first half of the queue and the other slot the
-                // rest, so the two run side by side instead of one waiting
-                const bool other_free = !s->slots[0].busy && !s->slots[1].busy;
-                const size_t take =
-                    other_free && s->pending.size() > 1 ? (s->pending.size() + 1) / 2 : s->pending.size();
-                size_t taken = 0;
+                // admit queued parts in arrival order while the aligned layout
+                // fits the slot and the batch stays within kBatchRows (a part
+                // that is larger runs alone): such a batch takes the one-launch
+                // split-K contraction, so a slot that frees under load takes a
+                // short batch and the backlog drains on every slot at once,
+                // instead of one slot taking the whole backlog into a long
+                // launch that every request in it waits for
+                long long taken_rows = 0;
                 std::vector<std::pair<const Model *, int>> per_model;
-                while (!s->pending.empty() && taken < take) {
+                while (!s->pending.empty() &&
+                       (taken_rows == 0 || taken_rows + s->pending.front()->n <= kBatchRows)) {
                     const ReqP &r = s->pending.front();
                     auto it = std::find_if(per_model.begin(), per_model.end(),
                                            [&](const std::pair<const Model *, int> &p) { return p.first == r->m; });
@@ -303,33 +326,41 @@ void worker_main(cufe_server *s)
                         per_model.push_back({r->m, r->n});
                     else
                         it->second += r->n;
+                    taken_rows += r->n;
                     sl.reqs.push_back(r);
                     s->pending.pop_front();
-                    ++taken;
                     if (!s->batching) break;
                 }
                 if (!sl.reqs.empty()) sl.busy = true;
             }
-            if (sl.busy) {
-                if (launch_batch(s, sl)) {
-                    sl.failed = true;
-                    sl.err = cufe_last_error();
-                }
-                const hipError_t e = hipEventRecord(sl.done, sl.st);
-                if (e != hipSuccess) {
-                    (void)hipStreamSynchronize(sl.st);
-                    retire(s, sl, e);
-                }
-                progressed = true;
+            if (!sl.busy) break;  // queue empty
+            if (launch_batch(s, sl)) {
+                sl.failed = true;
+                sl.err = cufe_last_error();
             }
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->inflight_bytes += sl.batch_bytes;
+                s->st_inflight_peak = std::max(s->st_inflight_peak, s->inflight_bytes);
+            }
+            const hipError_t e = hipEventRecord(sl.done, sl.st);
+            if (e != hipSuccess) {
+                (void)hipStreamSynchronize(sl.st);
+                retire(s, sl, e);
+            }
+            progressed = true;
         }
         if (progressed) continue;
-        bool any_busy = s->slots[0].busy || s->slots[1].busy;
-        if (!free_slot) {
-            // both slots run: query both again (yielding the core), so that
-            // whichever batch ends first is retired and its slot refilled at
-            // once -- a blocking wait on one of them can sit behind a larger
-            // batch while the other slot is already free
+        bool any_busy = false, any_free = false;
+        for (auto &slp : s->slots) {
+            any_busy |= slp->busy;
+            any_free |= !slp->busy;
+        }
+        if (!any_free) {
+            // every slot runs: query them all again (yielding the core), so
+            // that whichever batch ends first is retired and its slot refilled
+            // at once -- a blocking wait on one of them can sit behind a
+            // larger batch while another slot is already free
             std::this_thread::yield();
             continue;
         }
@@ -359,8 +390,8 @@ int submit_req(cufe_server *s, int model, int n, int l, const uint32_t *cl, cons
         return -1;
     }
     const Model *m = s->models[model].get();
-    if (n < 1 || round32(n) > s->cap) {
-        ipfe_set_error("cufe_server_submit: %d rows (slot capacity %d)", n, s->cap);
+    if (n < 1) {
+        ipfe_set_error("cufe_server_submit: %d rows", n);
         return -1;
     }
     if (l <= m->maxidx || l > s->l_max) {
@@ -371,24 +402,35 @@ int submit_req(cufe_server *s, int model, int n, int l, const uint32_t *cl, cons
         ipfe_set_error("cufe_server_submit: null buffer");
         return -1;
     }
-    auto r = std::make_shared<Req>();
-    r->model = model;
-    r->m = m;
-    r->n = n;
-    r->l = l;
-    r->cl = cl;
-    r->c0 = c0;
-    r->labels = labels;
-    r->kvalues = kvalues;
-    r->t_arrival = t_arrival;
+    // parts of at most one slot's capacity (a multiple of 64 rows)
+    const int part = s->cap;
+    const int nparts = (n + part - 1) / part;
+    auto tk = std::make_shared<Ticket>();
+    tk->t_arrival = t_arrival;
+    tk->parts_left = nparts;
+    std::vector<ReqP> parts;
+    for (int off = 0; off < n; off += part) {
+        auto r = std::make_shared<Req>();
+        r->tk = tk;
+        r->model = model;
+        r->m = m;
+        r->n = std::min(part, n - off);
+        r->l = l;
+        r->cl = cl + (size_t)off * kPoly;
+        r->c0 = c0 + (size_t)off * l * 3;
+        r->labels = labels + off;
+        r->kvalues = kvalues ? kvalues + (size_t)off * m->S : nullptr;
+        parts.push_back(std::move(r));
+    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        r->ticket = s->next_ticket++;
-        s->pending.push_back(r);
-        s->live[r->ticket] = r;
+        tk->id = s->next_ticket++;
+        for (auto &r : parts) s->pending.push_back(r);
+        s->live[tk->id] = tk;
+        if (nparts > 1) s->st_split++;
     }
     s->cv_submit.notify_one();
-    *ticket = r->ticket;
+    *ticket = tk->id;
     return 0;
 }
 
@@ -448,29 +490,37 @@ extern "C" int cufe_server_add_model(cufe_server *s, const cufe_svm_model *m, in
     return 0;
 }
 
-// Sizes both slots from the budget, allocates them and starts the worker.
+// Sizes the slots from the budget (a row capacity of up to max_batch_rows,
+// and 2 to 4 slots of it), allocates them and starts the worker.
 extern "C" int cufe_server_start(cufe_server *s)
 {
     if (!s || s->started || s->models.empty()) {
         ipfe_set_error("cufe_server_start: needs a created server with at least one model");
         return -1;
     }
+    // device bytes one batch row occupies: row store, staging (c_L and c0),
+    // key values of the largest model and the vote
     const size_t per_row = RowStore::bytes_per_row(s->l_max) + (kPoly + (size_t)s->l_max * 3) * 4 +
                            (size_t)s->S_max * 16 + 8;
-    // 64 MiB of headroom plus both slots' split-K scratch (fixed size)
-    const size_t reserve = ((size_t)64 << 20) + 2 * Scratch::split_bytes(s->max_batch, s->S_max);
-    if (s->budget < s->model_bytes + reserve + 2 * 64 * per_row) {
+    const size_t split = Scratch::split_bytes(s->max_batch, s->S_max);  // per slot, fixed size
+    const size_t headroom = (size_t)64 << 20;
+    if (s->budget < s->model_bytes + headroom + 2 * (split + 64 * per_row)) {
         ipfe_set_error("cufe_server_start: HBM budget %zu below models (%zu) + two 64-row slots (%zu)", s->budget,
-                       s->model_bytes, 2 * 64 * per_row);
+                       s->model_bytes, 2 * (split + 64 * per_row));
         return -1;
     }
-    size_t cap = (s->budget - s->model_bytes - reserve) / (2 * per_row);
+    const size_t avail = s->budget - s->model_bytes - headroom;
+    size_t cap = (avail / 2 - split) / per_row;  // the capacity two slots afford
     cap = std::min(cap, (size_t)((s->max_batch + 63) / 64 * 64));
     cap = std::min(cap, (size_t)1 << 20);
     s->cap = (int)(cap / 64 * 64);
-    s->slot_bytes = (size_t)s->cap * per_row;
+    s->row_bytes = per_row;
+    s->slot_bytes = (size_t)s->cap * per_row + split;
+    const int nslots = (int)std::max<size_t>(2, std::min<size_t>(4, avail / s->slot_bytes));
     const size_t stage_words = (size_t)s->cap * (kPoly + (size_t)s->l_max * 3);
-    for (Slot &sl : s->slots) {
+    for (int i = 0; i < nslots; ++i) {
+        s->slots.push_back(std::make_unique<Slot>());
+        Slot &sl = *s->slots.back();
         if (sl.rows.reserve(s->cap, s->l_max) || sl.sc.reserve(s->cap, s->S_max) || sl.d_stage.alloc(stage_words))
             return -1;
         HIP_OK(hipHostMalloc((void **)&sl.h_stage, stage_words * 4, hipHostMallocDefault));
@@ -485,7 +535,7 @@ extern "C" int cufe_server_start(cufe_server *s)
     // slot 0, with and without the split-K launch for small batches (a batch
     // may take either), so the contraction's block shapes are tuned here and
     // not on the first requests.
-    Slot &w = s->slots[0];
+    Slot &w = *s->slots[0];
     const int mode = contract_mode();
     for (int split_ok = 1; split_ok >= 0; --split_ok) {
         w.sc.split_ok = split_ok != 0;
@@ -530,7 +580,7 @@ extern "C" int cufe_server_wait(cufe_server *s, uint64_t ticket, double timeout_
         ipfe_set_error("cufe_server_wait: unknown ticket %llu", (unsigned long long)ticket);
         return -1;
     }
-    ReqP r = it->second;
+    TicketP r = it->second;
     auto ready = [&] { return r->status != 0; };
     if (timeout_ms < 0)
         s->cv_done.wait(lk, ready);
@@ -614,6 +664,25 @@ extern "C" int cufe_server_reset_stats(cufe_server *s)
     }
     std::lock_guard<std::mutex> lk(s->mu);
     s->st_requests = s->st_rows = s->st_batches = s->st_failed = s->st_batch_rows_max = 0;
+    s->st_split = 0;
+    s->st_inflight_peak = s->inflight_bytes;
+    return 0;
+}
+
+// Slots, requests split into parts, peak HBM bytes held by batches in flight
+// and device bytes one batch row occupies.
+extern "C" int cufe_server_info(cufe_server *s, int *nslots, long long *split_requests, long long *peak_inflight_bytes,
+                                long long *bytes_per_row)
+{
+    if (!s) {
+        ipfe_set_error("cufe_server_info: null server");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (nslots) *nslots = (int)s->slots.size();
+    if (split_requests) *split_requests = s->st_split;
+    if (peak_inflight_bytes) *peak_inflight_bytes = (long long)s->st_inflight_peak;
+    if (bytes_per_row) *bytes_per_row = (long long)s->row_bytes;
     return 0;
 }
 
@@ -629,6 +698,6 @@ extern "C" void cufe_server_destroy(cufe_server *s)
         s->cv_submit.notify_all();
         s->worker.join();
     }
-    for (Slot &sl : s->slots) free_slot_host(sl);
+    for (auto &sl : s->slots) free_slot_host(*sl);
     delete s;
 }

@@ -307,21 +307,27 @@ void cufe_encbatch_destroy(cufe_encbatch *eb);
 /*    functional key).  Requests carry rows in compact form (c_L and         */
 /*    coefficient 0 of c_0..c_{l-1}, as cufe_ctx_export_rows gives them); a */
 /*    worker thread packs queued requests of any loaded models into batches */
-/*    of up to the slot capacity, which the HBM budget fixes at start.      */
-/*    Results equal cufe_ctx_predict's for the same rows, alone or mixed.   */
+/*    of up to one slot's capacity.  The HBM budget fixes at start the slot  */
+/*    capacity (rows) and the number of slots (2 to 4) that run side by     */
+/*    side; a request larger than a slot runs as slot-sized parts.          */
+/*    Results equal cufe_ctx_predict's for the same rows, alone, mixed or   */
+/*    split.                                                                */
 /* ===================================================================== */
 
 typedef struct cufe_server cufe_server;
 
 /* ctx supplies the master keys and must outlive the server.  hbm_budget:
- * device bytes for keys and both batch slots (0 = 16 GiB); max_batch_rows:
- * largest batch (0 = 4096); batching 0 runs one request per batch. */
+ * device bytes for keys and the batch slots (0 = 16 GiB): a slot holds up to
+ * max_batch_rows rows (0 = 4096) if two such slots fit, fewer otherwise, and
+ * the budget left over buys up to two more slots; batching 0 runs one request
+ * (part) per batch. */
 int cufe_server_create(cufe_ctx *ctx, size_t hbm_budget, int max_batch_rows, int batching, cufe_server **out);
 /* Generates and keeps the functional keys of one model (before start). */
 int cufe_server_add_model(cufe_server *srv, const cufe_svm_model *model, int *model_id);
 /* Sizes and allocates the batch slots and starts the worker thread. */
 int cufe_server_start(cufe_server *srv);
-/* Queues n rows of l slots (model's largest feature < l <= key length):
+/* Queues n rows of l slots (model's largest feature < l <= key length); n
+ * above the slot capacity runs as parts of at most that many rows:
  * cl [n][3][4096], c0 [n][l][3].  labels [n] and kvalues [n][model l]
  * (optional) receive the results; every buffer stays the caller's and must
  * live until the request completes. */
@@ -339,6 +345,11 @@ int cufe_server_replay(cufe_server *srv, int nreq, const double *arrival_s, cons
 /* requests, rows, batches, failed, largest batch rows, slot capacity rows,
  * model key bytes, bytes per slot. */
 int cufe_server_stats(cufe_server *srv, long long *out8);
+/* Number of batch slots, requests split into parts, peak device bytes held by
+ * batches in flight (rows x bytes_per_row) and the device bytes one batch row
+ * occupies (row store, staging, key values, vote). */
+int cufe_server_info(cufe_server *srv, int *nslots, long long *split_requests, long long *peak_inflight_bytes,
+                     long long *bytes_per_row);
 /* Zeroes the request, row, batch, failure and largest-batch counters (after
  * an untimed warm-up, so that reported stats cover the timed requests only). */
 int cufe_server_reset_stats(cufe_server *srv);

@@ -3,9 +3,10 @@ alone, mixed with other models' requests, one request per batch, and under an
 open-loop Poisson replay -- equal, bit for bit, to cufe_ctx_predict on the same
 stored rows (which the predict tests pin to the oracle pair by pair).
 
-Also the server's error paths, forced from the host: unknown model, rows that
-exceed the slot, rows too short for the model, models added after start, and
-an HBM budget below what the models need."""
+Requests larger than a batch slot are split into slot-sized parts and return
+the same bits.  Also the server's error paths, forced from the host: unknown
+model, rows too short for the model, models added after start, and an HBM
+budget below what the models need."""
 import hashlib
 
 import numpy as np
@@ -226,9 +227,6 @@ def test_server_errors(gpu, env):
             srv.add_model(models[1][0])
         with pytest.raises(CufeError):
             srv.submit(5, *request(env, 0, 0, 1))
-        cl, c0 = request(env, 0, 0, 65)
-        with pytest.raises(CufeError):
-            srv.submit(0, cl, c0)
         cl, c0 = request(env, 0, 0, 2)
         with pytest.raises(CufeError):
             srv.submit(0, cl, np.ascontiguousarray(c0[:, :3]))  # 3 slots < the model's features
@@ -241,3 +239,30 @@ def test_server_errors(gpu, env):
     with pytest.raises(CufeError):
         small.start()
     small.close()
+
+
+def test_oversize_requests_are_split(gpu, env):
+    """With 64-row slots, requests of 96 and 80 rows run as slot-sized parts
+    (64 + 32, 64 + 16), interleaved with small requests of other models; every
+    request equals cufe_ctx_predict, labels and key values."""
+    _, pools, models, _ = env
+    srv = start_server(gpu, env, max_batch_rows=64)
+    try:
+        info = srv.info()
+        note(f"server info {info}")
+        assert 2 <= info["slots"] <= 4 and srv.stats()["cap_rows"] == 64
+        plan = [(0, 0, 96), (2, 3, 5), (1, 0, 80), (3, 1, 20), (0, 10, 64), (2, 0, 96)]
+        reqs = []
+        for mid, r0, n in plan:
+            cl, c0 = request(env, mid, r0, n)
+            reqs.append((srv.submit(mid, cl, c0, want_kvalues=True), mid, r0, n))
+        for t, mid, r0, n in reqs:
+            lab, kv, _ = srv.wait(t, 60000)
+            check(env, mid, r0, n, lab, kv)
+        st = srv.stats()
+        assert st["requests"] == len(plan) and st["failed"] == 0 and st["max_batch_rows"] <= 64
+        info = srv.info()
+        assert info["split_requests"] == 3
+        assert 0 < info["peak_inflight_bytes"] <= info["slots"] * 64 * info["bytes_per_row"]
+    finally:
+        srv.close()

@@ -540,8 +540,7 @@ def main():
         return r
 
     roof = roofline([1, 2], 4)
-    # labels of the kernels that use three row digits end in a letter and "3"
-    roof["limb0"] = roofline([0], 3 if i8 and re.search(r"/\d+[a-z]+3$", tuned.get("limb 0", "")) else 4)
+    roof["limb0"] = roofline([0], 3 if i8 and three_row_digits(tuned.get("limb 0", "")) else 4)
     roof["modular_macs_per_s"] = 3.0 * R * S * K * args.steps / (contract_ms * 1e-3)
     out = {
         "metric": "IPFE-SVM predictions/sec on 1 MI355X, batched vs plain (bit-exact); p99 ms",
@@ -598,6 +597,13 @@ def main():
     out["autotune"] = cf.autotune_report().strip().split("\n")  # every shape tuned, serving batches included
     detail = write_detail(out, args.detail)
     print(compact_line(out, detail), flush=True)
+
+
+def three_row_digits(label: str) -> bool:
+    """Whether a contraction variant label names a kernel that uses three row
+    digits: after the ring depth, the kernel letters end in "3", before the
+    optional tile-order suffix "gGRxKG"."""
+    return re.search(r"/\d+[a-z]+3(g\d+x\d+)?$", label) is not None
 
 
 def write_detail(out: dict, path: str) -> str:

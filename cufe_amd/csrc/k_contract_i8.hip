@@ -402,12 +402,28 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 // The main loop runs while the ring still has steps to load; the last
 // STAGES + 1 steps (at most) wait with the conservative vmcnt(0).
 // ---------------------------------------------------------------------------
+// Tile t of a launch -> (row tile tr, key tile tk).  Tiles run in groups of
+// gr row tiles; a group runs in sub-blocks of kg key tiles with the row tile
+// fastest.  The blocks one XCD runs at a time take neighbouring t, so they
+// share key tiles (consecutive t) and row tiles (t apart by gr): with gr x kg
+// about the blocks resident on an XCD, the row and key digits of the k steps
+// in flight stay in that XCD's L2.  gr = 1, kg = ntk is row-major order.
+__device__ __forceinline__ void tile_rc(int t, int ntk, int ntr, int gr, int kg, int &tr, int &tk)
+{
+    const int g = t / (gr * ntk), i = t - g * gr * ntk;
+    const int rg = min(gr, ntr - g * gr);  // row tiles of group g
+    const int sb = i / (rg * kg), j = i - sb * rg * kg;
+    tr = g * gr + j % rg;
+    tk = sb * kg + j / rg;
+}
+
 template <int WR, int WK, int NB, int STAGES, int DA, bool PT, int OCC, int KW>
 __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i *__restrict__ A,
                                                                     const v4i *__restrict__ B, int R, int S,
                                                                     int ksteps, long long a_stride,
                                                                     long long b_stride, uint32_t q,
-                                                                    I8Out out, int ntk, int ntiles)
+                                                                    I8Out out, int ntk, int ntiles, int gr,
+                                                                    int kg)
 {
     constexpr int W = WR * WK;
     constexpr int F = WR * DG_A + WK * KW * NB;
@@ -422,11 +438,13 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
     const int wr = w / WK, wk = w % WK;
     // This block's tiles t0, t0 + tstride, ... (nt of them).  The tiles are
     // split into 8 contiguous ranges, one per XCD (block b runs on XCD b mod
-    // 8), so the blocks of one XCD work on neighbouring tiles -- the key tiles
-    // of the same row tiles -- and share the row digits in that XCD's L2.
+    // 8), so the blocks of one XCD work on neighbouring tiles in tile_rc's
+    // order and share their digits in that XCD's L2.
     int t0, tstride, nt;
     if (PT) {
-        const int bx = gridDim.x / 8, per = (ntiles + 7) / 8;
+        // each XCD's share a whole number of rounds of its bx blocks, so the
+        // tiles its blocks run at once are one aligned run of tile_rc's order
+        const int bx = gridDim.x / 8, per = ((ntiles + 7) / 8 + bx - 1) / bx * bx;
         const int lo = (int)(blockIdx.x % 8) * per, hi = min(ntiles, lo + per);
         t0 = lo + (int)(blockIdx.x / 8);
         tstride = bx;
@@ -444,7 +462,8 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
     long long step[FW];
     const v4i *base[FW];
     auto set_base = [&](int j) {
-        const int t = t0 + j * tstride, tk = t % ntk, tr = t / ntk;
+        int tr, tk;
+        tile_rc(t0 + j * tstride, ntk, ntiles / ntk, gr, kg, tr, tk);
 #pragma unroll
         for (int u = 0; u < FW; ++u) {
             const int f = w + u * W;
@@ -536,7 +555,8 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
         rd_slot = rd_slot + 1 == STAGES ? 0 : rd_slot + 1;
     };
     auto store = [&](int j) {
-        const int t = t0 + j * tstride, tk = t % ntk, tr = t / ntk;
+        int tr, tk;
+        tile_rc(t0 + j * tstride, ntk, ntiles / ntk, gr, kg, tr, tk);
         const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
 #pragma unroll
         for (int kb = 0; kb < KW; ++kb) {
@@ -941,12 +961,14 @@ int launch_keys_digits(const double *kd, int S, int K, int spad, int ksteps, int
 // tile, the LDS ring depth (0: the register-staged double-buffered kernel),
 // the kernel, the row digits the MFMAs use (3: limb 0 only), and for the
 // interleaved kernels the waves per SIMD the registers are budgeted for and
-// the key blocks per wave (block tile: wr x wk kw blocks of 32).
+// the key blocks per wave (block tile: wr x wk kw blocks of 32), and their
+// tile order.
 enum I8Kind { I8_REG, I8_RING, I8_RING_K2, I8_HYBRID, I8_INTERLEAVED, I8_PERSISTENT };
 struct I8Variant {
     int wr, wk, stages, kind, da;
     int occ = 2;
     int kw = 1;
+    int gr = 0, kg = 0;  // tile order of the interleaved kernels: gr x kg groups (0: row-major)
 };
 static const I8Variant kVariants[] = {
     // register-staged, LDS rings and the hybrid (also the three-key-digit set)
@@ -963,6 +985,11 @@ static const I8Variant kVariants[] = {
     {4, 1, 3, I8_INTERLEAVED, 4, 2, 2}, {4, 1, 3, I8_INTERLEAVED, 3, 2, 2}, {2, 2, 4, I8_INTERLEAVED, 4, 2, 2},
     {2, 2, 4, I8_INTERLEAVED, 3, 2, 2}, {2, 2, 3, I8_INTERLEAVED, 4, 2, 2}, {2, 2, 5, I8_INTERLEAVED, 4, 2, 2},
     {2, 2, 5, I8_INTERLEAVED, 3, 2, 2},
+    // the same in grouped tile order (gr row tiles x kg key tiles)
+    {2, 2, 4, I8_INTERLEAVED, 4, 2, 2, 8, 8}, {2, 2, 5, I8_INTERLEAVED, 4, 2, 2, 8, 8},
+    {2, 2, 4, I8_INTERLEAVED, 3, 2, 2, 8, 8}, {4, 1, 3, I8_INTERLEAVED, 4, 2, 2, 4, 16},
+    {4, 1, 3, I8_INTERLEAVED, 3, 2, 2, 4, 16}, {1, 4, 4, I8_INTERLEAVED, 3, 3, 1, 16, 6},
+    {2, 2, 4, I8_INTERLEAVED, 4, 3, 1, 8, 12},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -975,7 +1002,8 @@ void i8_shape(int v, int *wr, int *wk)
 // "WRxWK" for the register-staged kernel; "WRxWK/STAGES" plus "k2" (two k
 // steps per barrier), "h" (hybrid), "i" (interleaved) or "ip" (interleaved,
 // persistent, each with "m": merged accumulators) for the others; "w" for
-// waves of two key blocks and "3" for three row digits
+// waves of two key blocks, "3" for three row digits and "gGRxKG" for the
+// grouped tile order
 const char *i8_variant_label(int v)
 {
     struct Labels {
@@ -991,6 +1019,9 @@ const char *i8_variant_label(int v)
                                                                 : "";
                 if (x.kind == I8_REG)
                     snprintf(s[i], sizeof s[i], "%dx%d", x.wr, x.wk);
+                else if (x.gr)
+                    snprintf(s[i], sizeof s[i], "%dx%d/%d%s%s%sg%dx%d", x.wr, x.wk * x.kw, x.stages, suffix,
+                             x.kw > 1 ? "w" : "", x.da == 3 ? "3" : "", x.gr, x.kg);
                 else
                     snprintf(s[i], sizeof s[i], "%dx%d/%d%s%s%s", x.wr, x.wk * x.kw, x.stages, suffix,
                              x.kw > 1 ? "w" : "", x.da == 3 ? "3" : "");
@@ -1116,6 +1147,8 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         return -1;
     }
     const I8Out D = {kval ? nullptr : D_out, crt_d0, crt_d1, kval};
+    const int gr = kVariants[variant].gr ? kVariants[variant].gr : 1;
+    const int kg = kVariants[variant].gr ? std::min(kVariants[variant].kg, ntk) : ntk;
 #define CI8(wr, wk, n) \
     hipLaunchKernelGGL((k_contract_i8<wr, wk, n>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
 #define CI8D(wr, wk, n, stg, kpb)                                                                                  \
@@ -1123,13 +1156,13 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
                        ntk, ntiles)
 #define CI8I(wr, wk, n, stg, da, occ, kw)                                                                            \
     hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, false, occ, kw>), grid, block, 0, st, a, b, R, S, ksteps, \
-                       as, bs, q, D, ntk, ntiles)
+                       as, bs, q, D, ntk, ntiles, gr, kg)
 #define CI8P(wr, wk, n, stg, da, occ, kw)                                                                            \
     hipLaunchKernelGGL(                                                                                              \
         (k_contract_i8_il<wr, wk, n, stg, da, true, occ, kw>),                                                       \
         dim3(persistent_grid(reinterpret_cast<const void *>(&k_contract_i8_il<wr, wk, n, stg, da, true, occ, kw>),   \
                              block.x, ntiles)),                                                                      \
-        block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles)
+        block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk, ntiles, gr, kg)
 #define CI8H(wk, n, stg)                                                                                             \
     hipLaunchKernelGGL((k_contract_i8_hyb<wk, n, stg>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, q, D, ntk,    \
                        ntiles)
@@ -1159,7 +1192,14 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         case 21: CI8I(2, 2, 2, 4, 3, 2, 2); break;
         case 22: CI8I(2, 2, 2, 3, 4, 2, 2); break;
         case 23: CI8I(2, 2, 2, 5, 4, 2, 2); break;
-        default: CI8I(2, 2, 2, 5, 3, 2, 2); break;  // variant 24
+        case 24: CI8I(2, 2, 2, 5, 3, 2, 2); break;
+        case 25: CI8I(2, 2, 2, 4, 4, 2, 2); break;
+        case 26: CI8I(2, 2, 2, 5, 4, 2, 2); break;
+        case 27: CI8I(2, 2, 2, 4, 3, 2, 2); break;
+        case 28: CI8I(4, 1, 2, 3, 4, 2, 2); break;
+        case 29: CI8I(4, 1, 2, 3, 3, 2, 2); break;
+        case 30: CI8I(1, 4, 2, 4, 3, 3, 1); break;
+        default: CI8I(2, 2, 2, 4, 4, 3, 1); break;  // variant 31
         }
     } else {
         switch (variant) {
@@ -1172,6 +1212,7 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         case 6: CI8H(4, 3, 3); break;
         case 8: CI8I(1, 4, 3, 4, 4, 2, 1); break;
         case 10: CI8I(1, 4, 3, 4, 3, 2, 1); break;
+        case 30: CI8I(1, 4, 3, 4, 3, 2, 1); break;
         default: CI8I(1, 4, 3, 3, 3, 2, 1); break;  // variant 13
         }
     }

@@ -94,3 +94,10 @@ def test_line_sheds_optional_blocks_when_too_long():
     assert len(text.encode()) <= bench.LINE_LIMIT
     line = json.loads(text)
     assert "roofline" in line and "cpu_baseline" in line and "p99_ms" in line
+
+
+def test_three_row_digit_labels():
+    for label in ("2x2/4im3", "2x4/4imw3", "4x2/3imw3g4x16", "2x4/4imw3g8x8", "2x4/4ipmw3g8x8"):
+        assert bench.three_row_digits(label), label
+    for label in ("2x2", "1x4/3", "2x4/5imw", "2x4/5imwg8x8", "1x4/2k2", "2x4/4ipmwg8x8", "2x2/4img8x12"):
+        assert not bench.three_row_digits(label), label

@@ -394,6 +394,14 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 // epilogue runs while the other's MFMAs keep the matrix pipe busy.  OCC is
 // the waves per SIMD the registers are budgeted for.
 //
+// RW = 2 gives each wave RW x KW = 2 x 2 blocks (64 x 64 outputs, 24 MFMAs
+// per step from 6 row and 4 key fragments with DA = 3): fewer LDS reads per
+// MFMA than 32 x 64, a 128 x 128 block tile (half the operand bytes per MFMA
+// from L2), and 16 accumulators of 16 registers -- all 256 accumulation
+// registers of a wave at one wave per SIMD.  (With DA = 4 the 20 accumulators
+// do not fit them, and the compiler moves accumulators between register files
+// on every step; only DA = 3, limb 0, runs this shape.)
+//
 // DA = 3 uses only row digits 0..2: valid when every row value lies in
 // [-8421504, 8355711] (digit 3 is then zero), which the row digit store
 // guarantees for limb 0 (q < 2^24).  The zero digit plane is still loaded, so
@@ -417,7 +425,7 @@ __device__ __forceinline__ void tile_rc(int t, int ntk, int ntr, int gr, int kg,
     tk = sb * kg + j / rg;
 }
 
-template <int WR, int WK, int NB, int STAGES, int DA, bool PT, int OCC, int KW>
+template <int WR, int WK, int NB, int STAGES, int DA, bool PT, int OCC, int KW, int RW = 1>
 __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i *__restrict__ A,
                                                                     const v4i *__restrict__ B, int R, int S,
                                                                     int ksteps, long long a_stride,
@@ -426,12 +434,13 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
                                                                     int kg)
 {
     constexpr int W = WR * WK;
-    constexpr int F = WR * DG_A + WK * KW * NB;
+    constexpr int FA = WR * RW * DG_A;  // row fragments per step (every row digit plane)
+    constexpr int F = FA + WK * KW * NB;
     static_assert(F % W == 0, "fragments must split evenly over the waves");
     static_assert(DA * NB >= 5 && DA <= DG_A, "the interleave needs five MFMAs per step");
     constexpr int FW = F / W;
     constexpr int STEP_V4 = F * FRAG_V4;
-    constexpr int NM = DA * NB * KW;  // MFMAs per step
+    constexpr int NM = RW * DA * NB * KW;  // MFMAs per step
     __shared__ v4i ring[STAGES * STEP_V4];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -467,11 +476,11 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
 #pragma unroll
         for (int u = 0; u < FW; ++u) {
             const int f = w + u * W;
-            if (f < WR * DG_A) {
-                base[u] = A + (size_t)(tr * WR + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4;
+            if (f < FA) {
+                base[u] = A + (size_t)(tr * WR * RW + f / DG_A) * a_stride + (f % DG_A) * FRAG_V4;
                 step[u] = DG_A * FRAG_V4;
             } else {
-                const int fb = f - WR * DG_A;
+                const int fb = f - FA;
                 base[u] = B + (size_t)(tk * WK * KW + fb / NB) * b_stride + (fb % NB) * FRAG_V4;
                 step[u] = NB * FRAG_V4;
             }
@@ -493,13 +502,17 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
             if (++it_j < nt) set_base(it_j);
         }
     };
-    const int a_off = wr * DG_A * FRAG_V4 + lane;
-    const int b_off = (WR * DG_A + wk * KW * NB) * FRAG_V4 + lane;
-    // per key block of the wave, one accumulator per weight 256^(d + e)
+    // row fragment x = (row block x / DA, digit x % DA) of the wave
+    const int a_off = wr * RW * DG_A * FRAG_V4 + lane;
+    auto a_frag = [&](int x) { return a_off + ((x / DA) * DG_A + x % DA) * FRAG_V4; };
+    const int b_off = (FA + wk * KW * NB) * FRAG_V4 + lane;
+    // per (row block, key block) of the wave, one accumulator per weight
+    // 256^(d + e)
     constexpr int NACC = DA + NB - 1;
-    v16i acc[KW * NACC];
+    constexpr int NBLK = RW * KW;
+    v16i acc[NBLK * NACC];
 #pragma unroll
-    for (int c = 0; c < KW * NACC; ++c) acc[c] = v16i{};
+    for (int c = 0; c < NBLK * NACC; ++c) acc[c] = v16i{};
 #pragma unroll
     for (int p = 0; p < STAGES; ++p)
         if (p < G) issue_next();
@@ -508,27 +521,28 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
     else
         wait_vmcnt<0>();
     asm volatile("s_barrier" ::: "memory");
-    v4i fa0[DA], fb0[KW * NB], fa1[DA], fb1[KW * NB];
+    constexpr int NFA = RW * DA;
+    v4i fa0[NFA], fb0[KW * NB], fa1[NFA], fb1[KW * NB];
 #pragma unroll
-    for (int d = 0; d < DA; ++d) fa0[d] = ring[a_off + d * FRAG_V4];
+    for (int x = 0; x < NFA; ++x) fa0[x] = ring[a_frag(x)];
 #pragma unroll
     for (int e = 0; e < KW * NB; ++e) fb0[e] = ring[b_off + e * FRAG_V4];
     int rd_slot = 1 % STAGES;  // slot of step g + 1
     // One step g.  In the last step the reads of "step G" fetch a slot no one
     // loads any more: in bounds, and never used.  Every wave runs the same
     // number of barriers.
-    auto body = [&](auto main, int g, v4i (&fa)[DA], v4i (&fb)[KW * NB], v4i (&na)[DA], v4i (&nb)[KW * NB]) {
+    auto body = [&](auto main, int g, v4i (&fa)[NFA], v4i (&fb)[KW * NB], v4i (&na)[NFA], v4i (&nb)[KW * NB]) {
         constexpr bool M = decltype(main)::value;
         const v4i *nx = ring + rd_slot * STEP_V4;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            // MFMA i: key block kb (alternating), row digit d, key digit e
-            // (reversed on odd d): two consecutive MFMAs never add into the
-            // same accumulator
-            const int kb = i % KW, j = i / KW;
+            // MFMA i: key block kb (alternating), row block rb, row digit d,
+            // key digit e (reversed on odd d): two consecutive MFMAs never
+            // add into the same accumulator
+            const int kb = i % KW, rb = (i / KW) % RW, j = i / (KW * RW);
             const int d = j / NB, e = (d & 1) ? NB - 1 - j % NB : j % NB;
-            const int c = kb * NACC + d + e;
-            acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[d], fb[kb * NB + e], acc[c], 0, 0, 0);
+            const int c = (rb * KW + kb) * NACC + d + e;
+            acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[rb * DA + d], fb[kb * NB + e], acc[c], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
             if (i == 1) {
                 if constexpr (M)
@@ -547,8 +561,8 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
             } else if (i >= 4) {
                 // row fragment i - 4; the last MFMA also takes any left over
 #pragma unroll
-                for (int d = 0; d < DA; ++d)
-                    if (d == i - 4 || (i == NM - 1 && d > i - 4)) na[d] = nx[a_off + d * FRAG_V4];
+                for (int x = 0; x < NFA; ++x)
+                    if (x == i - 4 || (i == NM - 1 && x > i - 4)) na[x] = nx[a_frag(x)];
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -557,15 +571,19 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
     auto store = [&](int j) {
         int tr, tk;
         tile_rc(t0 + j * tstride, ntk, ntiles / ntk, gr, kg, tr, tk);
-        const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
 #pragma unroll
-        for (int kb = 0; kb < KW; ++kb) {
-            const int key = tk * (32 * WK * KW) + (wk * KW + kb) * 32 + (lane & 31);
-            if (key >= S) continue;
-            // a persistent block stores with the next tile's fragments live
-            constexpr int LB = OCC < 3 ? 8 : PT ? 1 : 4;
-            i8_store_merged<DA, NB, LB>(reinterpret_cast<const v16i(&)[NACC]>(acc[kb * NACC]), row0, key, R, S, q,
-                                        out);
+        for (int rb = 0; rb < RW; ++rb) {
+            const int row0 = tr * (32 * WR * RW) + (wr * RW + rb) * 32 + (lane >> 5) * 4;
+#pragma unroll
+            for (int kb = 0; kb < KW; ++kb) {
+                const int key = tk * (32 * WK * KW) + (wk * KW + kb) * 32 + (lane & 31);
+                if (key >= S) continue;
+                // a persistent block stores with the next tile's fragments
+                // live, a 64 x 64 wave with the accumulators of 4 blocks
+                constexpr int LB = RW > 1 ? 1 : OCC < 3 ? 8 : PT ? 1 : 4;
+                i8_store_merged<DA, NB, LB>(reinterpret_cast<const v16i(&)[NACC]>(acc[(rb * KW + kb) * NACC]),
+                                            row0, key, R, S, q, out);
+            }
         }
     };
     // after step g of a persistent block: at the end of a tile, store it and
@@ -576,7 +594,7 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i 
             if (++c_ks < ksteps) return;
             store(c_j);
 #pragma unroll
-            for (int c = 0; c < KW * NACC; ++c) acc[c] = v16i{};
+            for (int c = 0; c < NBLK * NACC; ++c) acc[c] = v16i{};
             c_ks = 0;
             ++c_j;
         }
@@ -969,6 +987,7 @@ struct I8Variant {
     int occ = 2;
     int kw = 1;
     int gr = 0, kg = 0;  // tile order of the interleaved kernels: gr x kg groups (0: row-major)
+    int rw = 1;          // row blocks per wave (interleaved kernels)
 };
 static const I8Variant kVariants[] = {
     // register-staged, LDS rings and the hybrid (also the three-key-digit set)
@@ -990,20 +1009,23 @@ static const I8Variant kVariants[] = {
     {2, 2, 4, I8_INTERLEAVED, 3, 2, 2, 8, 8}, {4, 1, 3, I8_INTERLEAVED, 4, 2, 2, 4, 16},
     {4, 1, 3, I8_INTERLEAVED, 3, 2, 2, 4, 16}, {1, 4, 4, I8_INTERLEAVED, 3, 3, 1, 16, 6},
     {2, 2, 4, I8_INTERLEAVED, 4, 3, 1, 8, 12},
+    // interleaved, 64 x 64 per wave, one wave per SIMD (block tile 128 x 128),
+    // three row digits
+    {2, 2, 4, I8_INTERLEAVED, 3, 1, 2, 4, 8, 2}, {2, 2, 3, I8_INTERLEAVED, 3, 1, 2, 4, 8, 2},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 int i8_shape_count() { return kNumVariants; }
 void i8_shape(int v, int *wr, int *wk)
 {
-    *wr = kVariants[v].wr;
+    *wr = kVariants[v].wr * kVariants[v].rw;
     *wk = kVariants[v].wk * kVariants[v].kw;
 }
 // "WRxWK" for the register-staged kernel; "WRxWK/STAGES" plus "k2" (two k
 // steps per barrier), "h" (hybrid), "i" (interleaved) or "ip" (interleaved,
 // persistent, each with "m": merged accumulators) for the others; "w" for
-// waves of two key blocks, "3" for three row digits and "gGRxKG" for the
-// grouped tile order
+// waves of two key blocks ("q": 2 x 2 blocks), "3" for three row digits and
+// "gGRxKG" for the grouped tile order
 const char *i8_variant_label(int v)
 {
     struct Labels {
@@ -1020,8 +1042,8 @@ const char *i8_variant_label(int v)
                 if (x.kind == I8_REG)
                     snprintf(s[i], sizeof s[i], "%dx%d", x.wr, x.wk);
                 else if (x.gr)
-                    snprintf(s[i], sizeof s[i], "%dx%d/%d%s%s%sg%dx%d", x.wr, x.wk * x.kw, x.stages, suffix,
-                             x.kw > 1 ? "w" : "", x.da == 3 ? "3" : "", x.gr, x.kg);
+                    snprintf(s[i], sizeof s[i], "%dx%d/%d%s%s%sg%dx%d", x.wr * x.rw, x.wk * x.kw, x.stages, suffix,
+                             x.rw > 1 ? "q" : x.kw > 1 ? "w" : "", x.da == 3 ? "3" : "", x.gr, x.kg);
                 else
                     snprintf(s[i], sizeof s[i], "%dx%d/%d%s%s%s", x.wr, x.wk * x.kw, x.stages, suffix,
                              x.kw > 1 ? "w" : "", x.da == 3 ? "3" : "");
@@ -1127,7 +1149,7 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
                        variant, nb);
         return -1;
     }
-    const int WR = kVariants[variant].wr, WK = kVariants[variant].wk * kVariants[variant].kw;
+    const int WR = kVariants[variant].wr * kVariants[variant].rw, WK = kVariants[variant].wk * kVariants[variant].kw;
     // row tiles never reach past the 128 padding rows; key tiles stay within
     // the 128-key padding
     const int key_tile = 32 * WK, row_tile = 32 * WR;
@@ -1137,7 +1159,7 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
     }
     const int ntk = (S + key_tile - 1) / key_tile, ntr = (R + row_tile - 1) / row_tile;
     const int ntiles = ntk * ntr;
-    const dim3 grid(((ntiles + 7) / 8) * 8), block(64 * WR * kVariants[variant].wk);
+    const dim3 grid(((ntiles + 7) / 8) * 8), block(64 * kVariants[variant].wr * kVariants[variant].wk);
     const v4i *a = reinterpret_cast<const v4i *>(A);
     const v4i *b = reinterpret_cast<const v4i *>(B);
     const long long as = (long long)ksteps_a * DG_A * FRAG_V4, bs = (long long)ksteps * nb * FRAG_V4;
@@ -1157,6 +1179,9 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 #define CI8I(wr, wk, n, stg, da, occ, kw)                                                                            \
     hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, false, occ, kw>), grid, block, 0, st, a, b, R, S, ksteps, \
                        as, bs, q, D, ntk, ntiles, gr, kg)
+#define CI8Q(wr, wk, n, stg, da, occ, kw, rw)                                                                         \
+    hipLaunchKernelGGL((k_contract_i8_il<wr, wk, n, stg, da, false, occ, kw, rw>), grid, block, 0, st, a, b, R, S,     \
+                       ksteps, as, bs, q, D, ntk, ntiles, gr, kg)
 #define CI8P(wr, wk, n, stg, da, occ, kw)                                                                            \
     hipLaunchKernelGGL(                                                                                              \
         (k_contract_i8_il<wr, wk, n, stg, da, true, occ, kw>),                                                       \
@@ -1199,7 +1224,9 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
         case 28: CI8I(4, 1, 2, 3, 4, 2, 2); break;
         case 29: CI8I(4, 1, 2, 3, 3, 2, 2); break;
         case 30: CI8I(1, 4, 2, 4, 3, 3, 1); break;
-        default: CI8I(2, 2, 2, 4, 4, 3, 1); break;  // variant 31
+        case 31: CI8I(2, 2, 2, 4, 4, 3, 1); break;
+        case 32: CI8Q(2, 2, 2, 4, 3, 1, 2, 2); break;
+        default: CI8Q(2, 2, 2, 3, 3, 1, 2, 2); break;  // variant 33
         }
     } else {
         switch (variant) {
@@ -1221,6 +1248,7 @@ int launch_contract_i8(const int8_t *A, int ksteps_a, const int8_t *B, int kstep
 #undef CI8H
 #undef CI8I
 #undef CI8P
+#undef CI8Q
     HIP_OK(hipGetLastError());
     return 0;
 }

@@ -482,35 +482,47 @@ def main():
     # starts and ends with an idle GPU
     contract_ms = 0.0
     limb_ms = [0.0, 0.0, 0.0]
+    exact_pairs = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         labels, _, ms = ctx.predict(0, R)
         contract_ms += ms[0]
         limb_ms = [a + b for a, b in zip(limb_ms, ctx.contract_limb_ms())]
+        exact_pairs += ctx.contract_form()[1]
     elapsed = time.perf_counter() - t0
+    form = ctx.contract_form()[0]
     ms_per_step = 1e3 * elapsed / args.steps
     value = R * args.steps / elapsed
 
     # decryption check: every decrypted inner product equals the plaintext one
-    # mod p (the reference's rounding has no final mod p, so 0 may come out as p)
-    chk = min(R, 256)
+    # mod p (the reference's rounding has no final mod p, so 0 may come out as
+    # p), and the timed form's key values equal the per-limb exact form's
+    chk = min(R, 4096)
     _, kv, _ = ctx.predict(0, chk, want_kvalues=True)
     ip = x[:chk].astype(np.int64) @ sv.astype(np.int64).T
     dec_ok = float(np.mean((kv.astype(np.int64) % P_MOD) == ip))
     dec_p = int(np.sum(kv == P_MOD))
+    os.environ["CUFE_CONTRACT"] = "limbs"
+    try:
+        _, kv_limbs, _ = ctx.predict(0, chk, want_kvalues=True)
+    finally:
+        os.environ.pop("CUFE_CONTRACT", None)
+    dec_same = float(np.mean(kv == kv_limbs))
 
     chunk_rows = max(64, (1 << 30) // S)
     chunks = (R + chunk_rows - 1) // chunk_rows
     K = 4096 + (max(int(np.nonzero(sv.any(axis=0))[0].max()) + 1, 1) + 15) // 16 * 16  # key columns
     nb = ctx.contract_digits()
     i8 = nb > 0 and os.environ.get("CUFE_CONTRACT", "") not in ("sgpr", "lds")
-    # The dominant kernel: the contraction of limbs 1 and 2 (two of the three
-    # launches per chunk; limb 0, whose residues fit three int8 digits, runs a
-    # kernel with 3/4 of the MFMA work and is reported beside it).
+    # The dominant kernel: the fraction form's contraction (every limb in one
+    # launch per chunk), or, per limb, the contraction of limbs 1 and 2 (two of
+    # the three launches per chunk; limb 0, whose residues fit three int8
+    # digits, runs a kernel with 3/4 of the MFMA work and is reported beside it).
     tuned = {}
     for line in cf.autotune_report().splitlines():
         if line.startswith(f"S={S} R={R} ") and "-> variant " in line:
-            tuned["limb 0" if " limb 0:" in line else "limbs 1-2"] = line.split("-> variant ")[1].split("(")[1].rstrip(")")
+            which = "fraction" if "fraction form:" in line else "limb 0" if " limb 0:" in line else "limbs 1-2"
+            tuned[which] = line.split("-> variant ")[1].split("(")[1].rstrip(")")
     tfile = os.path.join(ROOT, "profiles", "k_contract_i8_traffic.json")
     tdata = {}
     if os.path.exists(tfile):
@@ -539,8 +551,22 @@ def main():
         r.update({"achieved": achieved, "frac": achieved / r["peak"], "traffic": traffic, "avg_launch_ms": avg})
         return r
 
-    roof = roofline([1, 2], 4)
-    roof["limb0"] = roofline([0], 3 if i8 and three_row_digits(tuned.get("limb 0", "")) else 4)
+    if form == 2:
+        launches = chunks * args.steps
+        avg = limb_ms[0] / launches
+        label = tuned.get("fraction")
+        ops = 2.0 * 17 * R * S * K / chunks  # 17 int8 digit GEMMs, one MAC = 2 ops
+        achieved = ops / (avg * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": tdata.get(label, {}).get("kernel", "k_contract_frac"), "variant": label,
+                "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                "work_per_launch": f"{R} rows x {S} keys x {K} columns x 17 digit products (int8 MFMA), "
+                                   f"all three limbs",
+                "achieved": achieved, "frac": achieved / INT8_PEAK_TOPS,
+                "traffic": tdata.get(label, {}).get("hbm_bytes_per_launch"), "avg_launch_ms": avg,
+                "exact_kernel_ms": limb_ms[1] / launches, "exact_pairs_per_step": exact_pairs / args.steps}
+    else:
+        roof = roofline([1, 2], 4)
+        roof["limb0"] = roofline([0], 3 if i8 and three_row_digits(tuned.get("limb 0", "")) else 4)
     roof["modular_macs_per_s"] = 3.0 * R * S * K * args.steps / (contract_ms * 1e-3)
     out = {
         "metric": "IPFE-SVM predictions/sec on 1 MI355X, batched vs plain (bit-exact); p99 ms",
@@ -553,7 +579,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 residues (exact int8-digit MFMA contraction)",
+        "dtype": "u32 residues (exact int8-digit MFMA contraction)" if form != 2 else
+                 "u32 residues (int8-digit MFMA contraction over 72-bit CRT fractions, exact by per-pair check)",
         "data": "synthetic",
         "config": {"workload": f"batched IPFE-SVM prediction: {R} sparse binary 123-dim rows x {S} SVs, "
                                f"N=4096, 3 RNS limbs, key length 785",
@@ -562,10 +589,13 @@ def main():
         "phases": {"setup_s": t_setup, "encrypt_s": t_enc, "encrypt_rows_per_s": R / t_enc,
                    "keygen_ms": t_keygen * 1e3,
                    "contract_ms_per_limb": [round(t / args.steps, 4) for t in limb_ms],
+                   "contract_form": {2: "fraction (all limbs in one launch; per-limb list: contraction, exact "
+                                        "kernel, 0)", 1: "int8 per limb", 0: "fp64"}[form],
                    "last_step_device_ms": {"contract": ms[0], "kvalue": ms[1],
                                                                          "vote": ms[2], "total": ms[3]},
                    "predictions_per_s_with_encryption": R / (t_enc + ms_per_step / 1e3)},
-        "decryption_check": {"pairs": chk * S, "fraction_equal_mod_p": dec_ok, "zero_decrypted_as_p": dec_p},
+        "decryption_check": {"pairs": chk * S, "fraction_equal_mod_p": dec_ok, "zero_decrypted_as_p": dec_p,
+                             "fraction_equal_per_limb_form": dec_same},
     }
     # serving right after the prediction leg, and the batched replay once more
     # after the single-operation, batched-encryption and plain legs (about 82 GB
@@ -636,7 +666,7 @@ def compact_line(out: dict, detail: str | None = None) -> str:
     roof = out.get("roofline")
     if roof:
         rk = ("bound", "kernel", "variant", "achieved", "peak", "unit", "frac", "traffic", "avg_launch_ms",
-              "work_per_launch")
+              "work_per_launch", "exact_kernel_ms", "exact_pairs_per_step")
         line["roofline"] = {k: _r(roof[k]) for k in rk if k in roof}
         if "limb0" in roof:
             line["roofline"]["limb0"] = {k: _r(roof["limb0"][k]) for k in ("variant", "achieved", "frac",
@@ -654,7 +684,7 @@ def compact_line(out: dict, detail: str | None = None) -> str:
     ph = out.get("phases")
     if ph:
         line["phases"] = {k: _r(ph[k]) for k in ("encrypt_rows_per_s", "keygen_ms", "contract_ms_per_limb",
-                                                 "predictions_per_s_with_encryption") if k in ph}
+                                                 "contract_form", "predictions_per_s_with_encryption") if k in ph}
     if "decryption_check" in out:
         line["decryption_check"] = out["decryption_check"]
     be = out.get("batched_encrypt")

@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
         "cufe_autotune_report": (C.c_int, [C.c_char_p, C.c_int]),
         "cufe_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
         "cufe_contract_variants": (C.c_int, [C.c_int, C.POINTER(C.c_ulonglong)]),
+        "cufe_contract_frac_variants": (C.c_int, []),
         "cufe_wire_size": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
         "cufe_wire_pack": (C.c_int, [C.c_int, _u32p, C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_void_p,
                                      C.c_size_t]),
@@ -333,6 +334,12 @@ def contract_variants(nb: int) -> list:
     return [v for v in range(n) if (m.value >> v) & 1]
 
 
+def frac_variants() -> int:
+    """Number of block shapes of the fraction-form contraction
+    (CUFE_FRAC_VARIANT=v, 0 <= v < this, forces one)."""
+    return int(lib().cufe_contract_frac_variants())
+
+
 def autotune_report() -> str:
     """Block-shape choices of the int8 contraction so far."""
     n = lib().cufe_autotune_report(None, 0)
@@ -539,6 +546,7 @@ def _ctx_sigs(lb):
         "cufe_ctx_export_rows": (C.c_int, [p, C.c_int, C.c_int, _u32p, _u32p]),
         "cufe_ctx_contract_digits": (C.c_int, [p]),
         "cufe_ctx_contract_limb_ms": (C.c_int, [p, C.POINTER(C.c_float)]),
+        "cufe_ctx_contract_form": (C.c_int, [p, C.POINTER(C.c_ulonglong)]),
         "cufe_svm_predict_gpu": (C.c_double, [C.POINTER(_SvmModelC), _u32p, _u32p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
@@ -622,6 +630,15 @@ class Context:
         ms = (C.c_float * 3)()
         _check(lib().cufe_ctx_contract_limb_ms(self._h, ms), "ctx_contract_limb_ms")
         return list(ms)
+
+    def contract_form(self):
+        """(form, exact_pairs) of the last predict call: form 2 = the fraction
+        form (every limb in one int8 contraction), 1 = int8 per limb, 0 = fp64;
+        exact_pairs = the (row, key) pairs its exact kernel recomputed."""
+        n = C.c_ulonglong()
+        form = lib().cufe_ctx_contract_form(self._h, C.byref(n))
+        _check(form if form < 0 else 0, "ctx_contract_form")
+        return form, n.value
 
     def model_info(self):
         kb = (C.c_int * 3)()

@@ -29,15 +29,12 @@
 #include <type_traits>
 
 #include "crt_dev.h"
+#include "i8_frag.h"
 #include "kernels_predict.h"
 
 namespace ipfe {
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
-
 constexpr int DG_A = 4;  // digits per row value (key values: NB = 2 or 3)
-constexpr int FRAG_V4 = 64;        // one fragment: 64 lanes x 16 bytes
 
 // Where a contraction writes: the limb's residues D[R][S], or -- for the last
 // limb, with kval set -- the rounded decryption of every pair, combined by CRT
@@ -271,15 +268,6 @@ __global__ __launch_bounds__(64 * WR * WK, (i8_min_waves<WR, WK, NB>())) void k_
 // s_waitcnt vmcnt(loads issued after step ks) and a barrier make step ks
 // visible to all waves.
 // ---------------------------------------------------------------------------
-typedef __attribute__((address_space(3))) void lds_void_t;
-typedef __attribute__((address_space(1))) void glb_void_t;
-
-template <int N>
-__device__ __forceinline__ void wait_vmcnt()
-{
-    static_assert(N >= 0 && N < 64, "vmcnt range");
-    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
-}
 
 template <int WR, int WK, int NB, int STAGES, int KPB>
 __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *__restrict__ A,
@@ -410,21 +398,6 @@ __global__ __launch_bounds__(64 * WR * WK, 2) void k_contract_i8_dma(const v4i *
 // The main loop runs while the ring still has steps to load; the last
 // STAGES + 1 steps (at most) wait with the conservative vmcnt(0).
 // ---------------------------------------------------------------------------
-// Tile t of a launch -> (row tile tr, key tile tk).  Tiles run in groups of
-// gr row tiles; a group runs in sub-blocks of kg key tiles with the row tile
-// fastest.  The blocks one XCD runs at a time take neighbouring t, so they
-// share key tiles (consecutive t) and row tiles (t apart by gr): with gr x kg
-// about the blocks resident on an XCD, the row and key digits of the k steps
-// in flight stay in that XCD's L2.  gr = 1, kg = ntk is row-major order.
-__device__ __forceinline__ void tile_rc(int t, int ntk, int ntr, int gr, int kg, int &tr, int &tk)
-{
-    const int g = t / (gr * ntk), i = t - g * gr * ntk;
-    const int rg = min(gr, ntr - g * gr);  // row tiles of group g
-    const int sb = i / (rg * kg), j = i - sb * rg * kg;
-    tr = g * gr + j % rg;
-    tk = sb * kg + j / rg;
-}
-
 template <int WR, int WK, int NB, int STAGES, int DA, bool PT, int OCC, int KW, int RW = 1>
 __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_i8_il(const v4i *__restrict__ A,
                                                                     const v4i *__restrict__ B, int R, int S,

@@ -51,6 +51,43 @@ bool i8_variant_ok(int variant, int nb, int s);
 // digit (|b| + 128 (1 + 256 + ...)) / 256^(nb-1) must stay within [-128, 127]
 constexpr unsigned int I8_KEY_MAX2 = 32512;
 constexpr unsigned int I8_KEY_MAX3 = 8322943;
+// Fraction-form contraction (k_contract_frac.hip): every limb in one int8
+// contraction over the CRT value's 72-bit fractions, key values written
+// directly; pairs too close to a rounding change are listed for k_frac_exact.
+struct FracConsts {
+    uint32_t u[3];            // (Q / q_j)^-1 mod q_j
+    uint64_t rlo[3], rhi[3];  // floor(2^128 / q_j)
+};
+struct FracOut {
+    uint32_t *kval;          // [R][S] key values
+    const uint64_t *margin;  // [S]: 1 + sum_k |B_v[k]|
+    uint32_t *amb;           // [amb_cap][2]: (row, key) of the pairs left to the exact kernel
+    uint32_t *amb_count;     // zeroed before the launch
+    int amb_cap;
+};
+FracConsts frac_consts();
+// bytes of the fraction digit store [(cap + 128) / 32][ksteps][9][64][16]
+size_t frac_digit_bytes(int cap, int ksteps);
+int launch_rows_frac(const uint32_t *cl, const uint32_t *c0, int src_l, int R, int r0, int cap, int ksteps,
+                     int8_t *dig, hipStream_t st);
+// keys: *neq set unless every coefficient is the same integer (within int16)
+// in all limbs; margin[v] = (1 + sum_k |kd[0][v][k]|) 2^widen, at most 2^62
+// (widen <= 62; > 0 only in tests, to send many pairs to the exact kernel);
+// k16 [S][K] = limb 0's values
+int launch_keys_frac(const double *kd, int S, int K, int widen, uint64_t *margin, int16_t *k16, int *neq,
+                     hipStream_t st);
+// A: fraction digits at the first row (a multiple of 32), B: limb 0's
+// two-digit key store
+int launch_contract_frac(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int R, int S, const FracOut &out,
+                         hipStream_t st, int variant);
+// the listed pairs from the balanced row store (A at the first row, limb j at
+// + j limb_stride) and the key values k16 [S][K]; *overflow set when the list
+// overflowed; *total (if given) += the listed pairs
+int launch_frac_exact(const int32_t *A, size_t limb_stride, int ldr, const int16_t *k16, int S, int K,
+                      const FracOut &out, int *overflow, unsigned long long *total, hipStream_t st);
+int frac_variant_count();
+void frac_shape(int variant, int *wr, int *wk);
+const char *frac_variant_label(int variant);
 int launch_rows_blocked(const int32_t *a32, int cap, int cap64, int ldr, int r0, int R, double *a64, hipStream_t st);
 int launch_crt_kvalue(const uint32_t *D, size_t n, uint32_t *kval, hipStream_t st);
 int launch_kvalue(const uint32_t *D, const uint32_t *c0, int l, const SparseY &y, int R, int S, uint32_t *kval,

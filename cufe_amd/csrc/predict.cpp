@@ -30,10 +30,12 @@ constexpr size_t kPoly = (size_t)3 * IPFE_N;
 namespace ipfe {
 
 // ---------------------------------------------------------------------------
-// Block-shape autotuner of the int8 contraction.  The first launch of a shape
-// (keys, log2 of the rows, k steps) times every block shape on the real
-// operands -- all produce the same exact residues -- and keeps the fastest;
-// later launches of that shape use it.  CUFE_I8_VARIANT=v forces shape v.
+// Block-shape autotuner of the int8 contractions.  The first launch of a
+// shape (keys, log2 of the rows, k steps, key digits, limb 0 or not; or the
+// fraction form) times every block shape on the real operands -- all produce
+// the same results -- and keeps the fastest; later launches of that shape use
+// it.  CUFE_I8_VARIANT=v forces shape v of the per-limb kernels,
+// CUFE_FRAC_VARIANT=v of the fraction form.
 // ---------------------------------------------------------------------------
 namespace {
 std::mutex g_tune_mu;
@@ -49,28 +51,29 @@ int ilog2(int x)
     }
     return r;
 }
-}  // namespace
 
-// crt_d0, crt_d1, kval (limb 2 only): write the CRT-rounded key values
-// instead of this limb's residues.
-int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
-                      uint32_t *D, hipStream_t st, const uint32_t *crt_d0, const uint32_t *crt_d1, uint32_t *kval)
+// The tuned variant of a shape, or -1 when the shape has not been tuned.
+int tuned_variant(const std::tuple<int, int, int> &key)
 {
-    // a forced variant runs on the limbs it is valid for (three-row-digit
-    // variants: limb 0 only); the others are tuned
-    const char *env = getenv("CUFE_I8_VARIANT");
-    if (env && *env && i8_variant_ok(atoi(env), nb, s))
-        return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, atoi(env), crt_d0, crt_d1, kval);
-    // limb 0 has its own variant set (three row digits)
-    const auto key = std::make_tuple(S, ilog2(R), (ksteps * 4 + nb) * 2 + (s == 0));
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tune.find(key);
-        if (it != g_tune.end())
-            return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, it->second, crt_d0, crt_d1, kval);
-    }
-    // events released on every path out
-    struct EventPair {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    return it == g_tune.end() ? -1 : it->second;
+}
+
+// Times every variant in `valid` with launch(v) (after one untimed launch of
+// valid[0] that loads the code objects) in two passes, the second in reverse
+// order -- the clock drifts as the GPU warms during the sweep, and one slow
+// launch (a clock change, another stream's work) does not decide the choice
+// -- and keeps each variant's faster launch.  The fastest is cached for
+// `key`; ties within 2% go to the variant with the most rows per block tile,
+// which re-reads the key operand from beyond L2 the fewest times.  Appends a
+// report line "<prefix> label ms ... -> variant v (label)".
+template <class Launch>
+int sweep_variants(const std::tuple<int, int, int> &key, const std::vector<int> &valid, Launch launch,
+                   const char *(*label)(int), void (*shape)(int, int *, int *), const std::string &prefix,
+                   hipStream_t st)
+{
+    struct EventPair {  // released on every path out
         hipEvent_t e[2] = {nullptr, nullptr};
         ~EventPair()
         {
@@ -80,57 +83,91 @@ int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps
     } ev;
     HIP_OK(hipEventCreate(&ev.e[0]));
     HIP_OK(hipEventCreate(&ev.e[1]));
-    // one untimed launch loads the code objects
-    if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, 0, crt_d0, crt_d1, kval)) return -1;
-    std::vector<float> times(i8_shape_count(), -1.0f);
-    // Two passes over the variants, the second in reverse order, and the
-    // faster launch of each counts: the clock drifts as the GPU warms during
-    // the sweep, and one slow launch (a clock change, another stream's work)
-    // does not decide the choice.
-    std::vector<int> valid;
-    for (int v = 0; v < i8_shape_count(); ++v)
-        if (i8_variant_ok(v, nb, s)) valid.push_back(v);
+    if (valid.empty() || launch(valid[0])) return -1;
+    std::map<int, float> times;
     for (int pass = 0; pass < 2; ++pass) {
         for (size_t n = 0; n < valid.size(); ++n) {
             const int v = pass == 0 ? valid[n] : valid[valid.size() - 1 - n];
             HIP_OK(hipEventRecord(ev.e[0], st));
-            if (launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v, crt_d0, crt_d1, kval)) return -1;
+            if (launch(v)) return -1;
             HIP_OK(hipEventRecord(ev.e[1], st));
             HIP_OK(hipEventSynchronize(ev.e[1]));
             float t = 0;
             HIP_OK(hipEventElapsedTime(&t, ev.e[0], ev.e[1]));
-            if (times[v] < 0 || t < times[v]) times[v] = t;
+            auto it = times.find(v);
+            if (it == times.end() || t < it->second) times[v] = t;
         }
     }
+    std::string line = prefix;
     char buf[96];
-    snprintf(buf, sizeof buf, "S=%d R=%d ksteps=%d key digits=%d limb%s:", S, R, ksteps, nb,
-             s == 0 ? " 0" : "s 1-2");
-    std::string line = buf;
-    int best = valid.empty() ? 0 : valid[0];
-    float best_ms = 1e30f;
+    int best = valid[0];
     for (int v : valid) {
-        snprintf(buf, sizeof buf, " %s %.3f", i8_variant_label(v), times[v]);
+        snprintf(buf, sizeof buf, " %s %.3f", label(v), times[v]);
         line += buf;
-        if (times[v] < best_ms) {
-            best_ms = times[v];
-            best = v;
-        }
+        if (times[v] < times[best]) best = v;
     }
-    // Ties within 2% go to the variant with the most rows per block tile: it
-    // re-reads the key operand from beyond L2 the fewest times.
+    const float best_ms = times[best];
     for (int v : valid) {
         if (times[v] > 1.02f * best_ms) continue;
         int wr, wk, bwr, bwk;
-        i8_shape(v, &wr, &wk);
-        i8_shape(best, &bwr, &bwk);
+        shape(v, &wr, &wk);
+        shape(best, &bwr, &bwk);
         if (wr > bwr || (wr == bwr && times[v] < times[best])) best = v;
     }
-    snprintf(buf, sizeof buf, " ms -> variant %d (%s)\n", best, i8_variant_label(best));
+    snprintf(buf, sizeof buf, " ms -> variant %d (%s)\n", best, label(best));
     line += buf;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = best;
     g_tune_log += line;
+    return 0;
+}
+}  // namespace
+
+// crt_d0, crt_d1, kval (limb 2 only): write the CRT-rounded key values
+// instead of this limb's residues.
+int contract_i8_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int nb, int R, int S, int s,
+                      uint32_t *D, hipStream_t st, const uint32_t *crt_d0, const uint32_t *crt_d1, uint32_t *kval)
+{
+    auto launch = [&](int v) {
+        return launch_contract_i8(A, ksteps_a, B, ksteps, nb, R, S, s, D, st, v, crt_d0, crt_d1, kval);
+    };
+    // a forced variant runs on the limbs it is valid for (three-row-digit
+    // variants: limb 0 only); the others are tuned
+    const char *env = getenv("CUFE_I8_VARIANT");
+    if (env && *env && i8_variant_ok(atoi(env), nb, s)) return launch(atoi(env));
+    // limb 0 has its own variant set (three row digits)
+    const auto key = std::make_tuple(S, ilog2(R), (ksteps * 4 + nb) * 2 + (s == 0));
+    const int v = tuned_variant(key);
+    if (v >= 0) return launch(v);
+    std::vector<int> valid;
+    for (int x = 0; x < i8_shape_count(); ++x)
+        if (i8_variant_ok(x, nb, s)) valid.push_back(x);
+    char buf[96];
+    snprintf(buf, sizeof buf, "S=%d R=%d ksteps=%d key digits=%d limb%s:", S, R, ksteps, nb, s == 0 ? " 0" : "s 1-2");
+    if (sweep_variants(key, valid, launch, i8_variant_label, i8_shape, buf, st)) return -1;
     return 0;  // D holds the last variant's (identical) result
+}
+
+// The fraction form over rows A (fraction digits) and limb 0's key digits B.
+// The sweep's launches write the same key values and list the same pairs (the
+// count is cleared before each launch).
+int contract_frac_tuned(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int R, int S, const FracOut &out,
+                        hipStream_t st)
+{
+    auto launch = [&](int v) {
+        return launch_fill_i32((int *)out.amb_count, 0, 1, st) ||
+               launch_contract_frac(A, ksteps_a, B, ksteps, R, S, out, st, v);
+    };
+    const char *env = getenv("CUFE_FRAC_VARIANT");
+    if (env && *env && atoi(env) >= 0 && atoi(env) < frac_variant_count()) return launch(atoi(env));
+    const auto key = std::make_tuple(S, ilog2(R), -ksteps);
+    const int v = tuned_variant(key);
+    if (v >= 0) return launch(v);
+    std::vector<int> valid;
+    for (int x = 0; x < frac_variant_count(); ++x) valid.push_back(x);
+    char buf[96];
+    snprintf(buf, sizeof buf, "S=%d R=%d ksteps=%d fraction form:", S, R, ksteps);
+    return sweep_variants(key, valid, launch, frac_variant_label, frac_shape, buf, st);
 }
 
 int contract_mode()
@@ -138,7 +175,8 @@ int contract_mode()
     const char *m = getenv("CUFE_CONTRACT");
     if (m && strcmp(m, "lds") == 0) return 0;
     if (m && strcmp(m, "sgpr") == 0) return 1;
-    return 2;
+    if (m && strcmp(m, "limbs") == 0) return 2;
+    return 3;
 }
 
 // ---------------------------------------------------------------------------
@@ -242,11 +280,28 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
     nb = mx3 <= I8_KEY_MAX2 ? 2 : (mx3 <= I8_KEY_MAX3 ? 3 : 0);
     i8 = nb > 0;
     kdig.reset();
+    margin.reset();
+    k16.reset();
+    frac = false;
     if (i8) {
         spad = (nS + 127) / 128 * 128;
         ksteps = (kk + 31) / 32;
         if (kdig.alloc(3 * keys_digit_limb_bytes(spad, ksteps, nb))) return -1;
         if (launch_keys_digits(kd.get(), nS, kk, spad, ksteps, nb, kdig.get(), st) || sync_stream(st)) return -1;
+    }
+    if (i8 && nb == 2 && fz) {
+        // the fraction form needs keys that are the same integers in every limb
+        // (CUFE_FRAC_WIDEN=w, for tests: margins 2^w times wider, so that many
+        // pairs go to the exact kernel)
+        DevBuf<int> neq;
+        int h_neq = 1;
+        const char *wv = getenv("CUFE_FRAC_WIDEN");
+        if (margin.alloc(nS) || k16.alloc((size_t)nS * kk) || neq.alloc(1) || launch_fill_i32(neq.get(), 0, 1, st) ||
+            launch_keys_frac(kd.get(), nS, kk, wv ? atoi(wv) : 0, margin.get(), k16.get(), neq.get(), st))
+            return -1;
+        HIP_OK(hipMemcpyAsync(&h_neq, neq.get(), 4, hipMemcpyDeviceToHost, st));
+        if (sync_stream(st)) return -1;
+        frac = h_neq == 0;
     }
     S = nS;
     nr_class = m->nr_class;
@@ -257,19 +312,22 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
     return 0;
 }
 
-size_t Model::bytes() const { return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8) + kdig.n; }
+size_t Model::bytes() const
+{
+    return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8) + kdig.n + margin.n * 8 + k16.n * 2;
+}
 
 // ---------------------------------------------------------------------------
 // RowStore
 // ---------------------------------------------------------------------------
-size_t RowStore::bytes_per_row(int l)
+size_t RowStore::bytes_per_row(int l, bool fractions)
 {
     const size_t ldr = IPFE_N + ((l + 15) / 16) * 16;
     const size_t kd = IPFE_N + ((l + 31) / 32) * 32;
-    return 3 * ldr * (4 + 8) + 3 * kd * 4 + (size_t)l * 12;
+    return 3 * ldr * (4 + 8) + 3 * kd * 4 + (size_t)l * 12 + (fractions ? 9 * kd : 0);
 }
 
-int RowStore::reserve(int cap_rows, int len)
+int RowStore::reserve(int cap_rows, int len, bool fractions)
 {
     if (cap_rows < 1 || len < 1 || len > IPFE_L) {
         ipfe_set_error("row store: cap %d / l %d invalid", cap_rows, len);
@@ -283,6 +341,11 @@ int RowStore::reserve(int cap_rows, int len)
         return -1;
     HIP_OK(hipMemset(A.get(), 0, (size_t)3 * cp * ld * 4));
     HIP_OK(hipMemset(Adig.get(), 0, Adig.n));
+    Fdig.reset();
+    if (fractions) {
+        if (Fdig.alloc(frac_digit_bytes(cp, ks))) return -1;
+        HIP_OK(hipMemset(Fdig.get(), 0, Fdig.n));
+    }
     a64_ok.assign((size_t)cp / 64, 0);
     cap = cp;
     l = len;
@@ -301,6 +364,7 @@ int RowStore::put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev,
     if (n == 0) return 0;
     if (launch_rows_put(cl_dev, c0_dev, src_l, n, r0, cap, ldr, l, ksteps, A.get(), c0.get(), Adig.get(), st))
         return -1;
+    if (Fdig.get() && launch_rows_frac(cl_dev, c0_dev, src_l, n, r0, cap, ksteps, Fdig.get(), st)) return -1;
     for (int b = r0 / 64; b <= (r0 + n - 1) / 64; ++b) a64_ok[b] = 0;
     return 0;
 }
@@ -331,6 +395,10 @@ int RowStore::ensure_a64(int r0, int R, hipStream_t st) const
 // under load the split launch slows the other slot's contraction more than it
 // gains.
 constexpr int kSplitMaxRows = 256;
+// pairs the fraction form can leave to its exact kernel per chunk (8 MiB of
+// list); a chunk with more sets the overflow flag and the caller runs the
+// prediction again per limb
+constexpr int kAmbCap = 1 << 20;
 constexpr size_t kSplitMaxPairs = (size_t)1 << 19;
 
 size_t Scratch::split_bytes(int rows, int S)
@@ -369,7 +437,7 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
     Device *dev = device();
     if (!dev) return -1;
     const int S = m.S;
-    if (mode == 2 && m.i8 && row0 % 32 != 0 && nrows > 0) {
+    if (mode >= 2 && m.i8 && row0 % 32 != 0 && nrows > 0) {
         // The int8 kernel reads rows in 32-row blocks: predict from the block
         // start and keep the requested rows (results are per row, so the extra
         // rows change nothing).
@@ -389,12 +457,20 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
     if (tm) tm->ms[0] = tm->ms[1] = tm->ms[2] = tm->limb_ms[0] = tm->limb_ms[1] = tm->limb_ms[2] = 0;
     // the flag is raised only by the separate key-value fold (unfused keys)
     if (!m.fused && launch_fill_i32(sc.flag.get(), 0, 1, st)) return -1;
+    const bool frac_ok = mode >= 3 && m.i8 && m.frac && rs.Fdig.get() && !getenv("CUFE_NO_FRAC");
+    if (frac_ok) {
+        if ((!sc.amb.get() && sc.amb.alloc((size_t)2 * kAmbCap)) || (!sc.amb_count.get() && sc.amb_count.alloc(1)) ||
+            (!sc.amb_overflow.get() && sc.amb_overflow.alloc(1)) || (!sc.amb_total.get() && sc.amb_total.alloc(1)))
+            return -1;
+        if (launch_fill_i32(sc.amb_overflow.get(), 0, 1, st) || launch_fill_i32((int *)sc.amb_total.get(), 0, 2, st))
+            return -1;
+    }
     for (int r0 = row0; r0 < row0 + nrows; r0 += chunk) {
         const int R = std::min(chunk, row0 + nrows - r0);
         if (tm) HIP_OK(hipEventRecord(tm->ev[0], st));
         // every limb on the int8 path: the last one's epilogue also does the
         // CRT and rounding of the fused keys' key values
-        const bool i8 = mode == 2 && m.i8 && r0 % 32 == 0;
+        const bool i8 = mode >= 2 && m.i8 && r0 % 32 == 0;
         const bool fuse_crt = i8 && m.fused && !getenv("CUFE_NO_FUSED_CRT");
         // a small row range: all limbs and slices of the k steps in one launch
         const bool split = fuse_crt && sc.split_ok && m.nb == 2 && R <= kSplitMaxRows &&
@@ -414,7 +490,21 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
                 HIP_OK(hipEventRecord(tm->ev[5], st));
             }
         }
-        for (int s = 0; s < 3 && !split; ++s) {
+        // every limb in one contraction over the CRT values' fractions, then
+        // the listed pairs exactly
+        const bool frac = !split && i8 && frac_ok;
+        if (frac) {
+            const FracOut fo = {sc.kval.get(), m.margin.get(), sc.amb.get(), sc.amb_count.get(), kAmbCap};
+            if (contract_frac_tuned(rs.Fdig.get() + (size_t)(r0 / 32) * rs.ksteps * 9 * 1024, rs.ksteps, m.kdig.get(),
+                                    m.ksteps, R, S, fo, st))
+                return -1;
+            if (tm) HIP_OK(hipEventRecord(tm->ev[4], st));
+            if (launch_frac_exact(rs.A.get() + (size_t)r0 * rs.ldr, (size_t)rs.cap * rs.ldr, rs.ldr, m.k16.get(), S,
+                                  m.kkey, fo, sc.amb_overflow.get(), sc.amb_total.get(), st))
+                return -1;
+            if (tm) HIP_OK(hipEventRecord(tm->ev[5], st));
+        }
+        for (int s = 0; s < 3 && !split && !frac; ++s) {
             const int32_t *As = rs.A.get() + ((size_t)s * rs.cap + r0) * rs.ldr;
             const double *Bs = m.kd.get() + (size_t)s * S * m.kkey;
             uint32_t *Ds = sc.D.get() + (size_t)s * R * S;
@@ -439,9 +529,9 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
             if (tm && s < 2) HIP_OK(hipEventRecord(tm->ev[4 + s], st));
         }
         if (tm) HIP_OK(hipEventRecord(tm->ev[1], st));
-        if (fuse_crt) {
+        if (fuse_crt || frac) {
             // key values already written by limb 2's contraction (or the split
-            // launch's finish kernel)
+            // launch's finish kernel, or the fraction form)
         } else if (m.fused) {
             if (launch_crt_kvalue(sc.D.get(), (size_t)R * S, sc.kval.get(), st)) return -1;
         } else if (launch_kvalue(sc.D.get(), rs.c0.get() + (size_t)r0 * rs.l * 3, rs.l, m.y, R, S, sc.kval.get(),
@@ -570,6 +660,10 @@ struct cufe_ctx {
     Scratch scratch;
     hipEvent_t ev[10] = {};
     float limb_ms[3] = {};  // contraction per limb in the last predict call
+    // form of the last predict call's contraction (1: int8 per limb, 2: the
+    // fraction form, 0: fp64) and the pairs its exact kernel recomputed
+    int form = 0;
+    unsigned long long exact_pairs = 0;
 };
 
 namespace ipfe {
@@ -683,7 +777,7 @@ extern "C" int cufe_ctx_rows_reserve(cufe_ctx *c, int cap, int l)
         ipfe_set_error("cufe_ctx_rows_reserve: l %d invalid for key length %d", l, c ? c->l_key : 0);
         return -1;
     }
-    if (c->rows.reserve(cap, l)) return -1;
+    if (c->rows.reserve(cap, l, true)) return -1;
     c->nrows = 0;
     return 0;
 }
@@ -736,13 +830,33 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
     for (int i = 0; i < 4; ++i) tm.ev[i] = c->ev[i];
     tm.ev[4] = c->ev[8];
     tm.ev[5] = c->ev[9];
+    int mode = contract_mode();
+    const bool frac = mode >= 3 && c->model.i8 && c->model.frac && c->rows.Fdig.get() && !getenv("CUFE_NO_FRAC");
     HIP_OK(hipEventRecord(c->ev[4], st));
-    if (predict_rows(c->model, c->rows, row0, nrows, c->scratch, chunk_rows, contract_mode(), labels, kvalues, &tm, st))
+    if (predict_rows(c->model, c->rows, row0, nrows, c->scratch, chunk_rows, mode, labels, kvalues, &tm, st))
         return -1;
-    int flag = 0;
+    int flag = 0, overflow = 0;
+    unsigned long long exact_pairs = 0;
     if (!c->model.fused) HIP_OK(hipMemcpyAsync(&flag, c->scratch.flag.get(), 4, hipMemcpyDeviceToHost, st));
+    if (frac) {
+        HIP_OK(hipMemcpyAsync(&overflow, c->scratch.amb_overflow.get(), 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(&exact_pairs, c->scratch.amb_total.get(), 8, hipMemcpyDeviceToHost, st));
+    }
     HIP_OK(hipEventRecord(c->ev[5], st));
     HIP_OK(hipEventSynchronize(c->ev[5]));
+    if (overflow) {
+        // more pairs near a rounding change than the exact kernel's list holds
+        // in one chunk: the whole call again per limb
+        if (tm.pending) tm.pending = false;
+        mode = 2;
+        HIP_OK(hipEventRecord(c->ev[4], st));
+        if (predict_rows(c->model, c->rows, row0, nrows, c->scratch, chunk_rows, mode, labels, kvalues, &tm, st))
+            return -1;
+        HIP_OK(hipEventRecord(c->ev[5], st));
+        HIP_OK(hipEventSynchronize(c->ev[5]));
+    }
+    c->form = c->model.i8 && mode >= 2 ? (frac && !overflow ? 2 : 1) : 0;
+    c->exact_pairs = exact_pairs;
     if (tm.pending && predict_timers_collect(&tm)) return -1;
     float tot = 0;
     HIP_OK(hipEventElapsedTime(&tot, c->ev[4], c->ev[5]));
@@ -759,6 +873,19 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
         return -1;
     }
     return 0;
+}
+
+// Form of the last cufe_ctx_predict call's contraction: 2 = the fraction form
+// (every limb in one int8 contraction), 1 = int8 per limb, 0 = fp64; and the
+// (row, key) pairs its exact kernel recomputed (fraction form).
+extern "C" int cufe_ctx_contract_form(const cufe_ctx *c, unsigned long long *exact_pairs)
+{
+    if (!c) {
+        ipfe_set_error("cufe_ctx_contract_form: null context");
+        return -1;
+    }
+    if (exact_pairs) *exact_pairs = c->exact_pairs;
+    return c->form;
 }
 
 // Contraction time per limb (ms, summed over chunks) of the last
@@ -799,6 +926,9 @@ extern "C" int cufe_contract_variants(int nb, unsigned long long *valid_mask)
     if (valid_mask) *valid_mask = m;
     return i8_shape_count();
 }
+
+// Number of block shapes of the fraction-form contraction (CUFE_FRAC_VARIANT).
+extern "C" int cufe_contract_frac_variants(void) { return frac_variant_count(); }
 
 // Key digits of the int8 contraction for the loaded model (2 or 3), or 0 when
 // its key coefficients are too large and the fp64 kernels run.

@@ -34,6 +34,12 @@ struct Model {
     bool i8 = false;
     int nb = 0, spad = 0, ksteps = 0;
     DevBuf<int8_t> kdig;
+    // the fraction-form contraction applies (two key digits, fused keys, every
+    // key coefficient the same integer in the three limbs); margin [S] holds
+    // 1 + sum_k |B_v[k]| of each key
+    bool frac = false;
+    DevBuf<uint64_t> margin;
+    DevBuf<int16_t> k16;  // [S][kkey] key values for the fraction form's exact kernel
 
     int build(const uint32_t *msk_dev, int l_key, const cufe_svm_model *m, hipStream_t st);
     size_t bytes() const;
@@ -44,24 +50,27 @@ struct Model {
 // same rows blocked for the scalar-operand contraction ([3][(cap+64)/16][ldr][16]
 // fp64), as int8 digit fragments for the matrix-core contraction
 // ([3][(cap+128)/32][ksteps][4][64][16], ksteps = (N + l rounded up to 32) / 32),
-// and [cap][l][3] canonical coefficient-0 residues.
+// optionally as the nine fraction digits of each column's CRT value
+// ([(cap+128)/32][ksteps][9][64][16], for the fraction-form contraction), and
+// [cap][l][3] canonical coefficient-0 residues.
 struct RowStore {
     int l = 0, cap = 0, ldr = 0, ksteps = 0;
     DevBuf<int32_t> A;
     DevBuf<int8_t> Adig;
+    DevBuf<int8_t> Fdig;  // empty unless reserved with fractions
     DevBuf<uint32_t> c0;
     // fp64 blocked copy, built on demand (only the fp64 fallback kernel reads
     // it): allocated at first use, converted per 64-row block when stale
     mutable DevBuf<double> A64;
     mutable std::vector<uint8_t> a64_ok;
 
-    int reserve(int cap, int l);
+    int reserve(int cap, int l, bool fractions = false);
     // rows [r0, r0+R) of the fp64 blocked copy current
     int ensure_a64(int r0, int R, hipStream_t st) const;
     // rows [r0, r0+n) from device buffers: cl [n][3][N] canonical, c0
     // [n][src_l][3] with src_l <= l slots (default l); slots past src_l read 0
     int put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev, hipStream_t st, int src_l = -1);
-    static size_t bytes_per_row(int l);
+    static size_t bytes_per_row(int l, bool fractions = false);
 };
 
 struct Scratch {
@@ -69,6 +78,11 @@ struct Scratch {
     DevBuf<uint32_t> Dpart;  // split-K slices of small row ranges
     DevBuf<double> labels;
     DevBuf<int> flag;
+    // fraction form: the pairs left to the exact kernel, their count (per
+    // chunk), a list-overflow flag and the listed total (per predict_rows)
+    DevBuf<uint32_t> amb, amb_count;
+    DevBuf<int> amb_overflow;
+    DevBuf<unsigned long long> amb_total;
     // small row ranges may use the split-K launch (a server turns it off while
     // its other slot runs: there it would slow that slot's contraction)
     bool split_ok = true;
@@ -82,8 +96,10 @@ struct Scratch {
 // that the scratch stays within `scratch_rows` rows.  ms (optional, 3 floats,
 // only when timing events are passed): contraction / key-value / vote time.
 // ev[0..3]: before the contraction, after it, after the key values, after
-// the vote; ev[4], ev[5]: after the contraction of limb 0 and of limb 1.
-// ms: contraction, key values, vote; limb_ms: the contraction per limb.
+// the vote; ev[4], ev[5]: after the contraction of limb 0 and of limb 1 (the
+// fraction form: after its contraction and after the exact kernel).
+// ms: contraction, key values, vote; limb_ms: the contraction per limb (the
+// fraction form: its contraction, the exact kernel, 0).
 struct PredictTimers {
     hipEvent_t ev[6];
     float ms[3];
@@ -100,10 +116,14 @@ int predict_rows(const Model &m, const RowStore &rs, int r0, int R, Scratch &sc,
 int encrypt_rows_compact(const uint32_t *mpk_dev, int l_key, RowStore &rs, int r0, const uint32_t *x, int n,
                          const unsigned char seed2[32], const unsigned char seed3[32], int nonce0, hipStream_t st);
 
-// Contraction kernel choice: 2 = int8 matrix cores (default), 1 = fp64
-// scalar-operand (CUFE_CONTRACT=sgpr), 0 = fp64 LDS tiles (CUFE_CONTRACT=lds).
-// The int8 kernel needs key coefficients within two digits and rows starting
-// on a 32-row boundary; otherwise the fp64 kernels run.
+// Contraction kernel choice: 3 = the fraction form where it applies, else the
+// int8 matrix cores per limb (default), 2 = int8 per limb
+// (CUFE_CONTRACT=limbs), 1 = fp64 scalar-operand (CUFE_CONTRACT=sgpr), 0 = fp64
+// LDS tiles (CUFE_CONTRACT=lds).  The int8 kernels need key coefficients
+// within two (fraction form) or three digits and rows starting on a 32-row
+// boundary; otherwise the fp64 kernels run.  A fraction-form pass whose list
+// of pairs for the exact kernel overflowed sets Scratch::amb_overflow; the
+// caller then runs the pass again per limb.
 int contract_mode();
 
 }  // namespace ipfe

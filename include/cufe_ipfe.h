@@ -253,8 +253,19 @@ int cufe_ctx_predict(cufe_ctx *ctx, int row0, int nrows, double *labels, uint32_
  * cufe_ctx_predict call (limb 0 may run a cheaper kernel: its residues fit
  * three int8 digits).  Chunks of at most 256 rows (and 2^19 row-key pairs)
  * run the three limbs in one split-K launch, whose whole time counts as
- * limb 0's. */
+ * limb 0's.  In the fraction form: ms[0] = its contraction (all limbs),
+ * ms[1] = its exact kernel, ms[2] = 0. */
 int cufe_ctx_contract_limb_ms(const cufe_ctx *ctx, float *ms);
+/* Form of the last cufe_ctx_predict call's contraction: 2 = the fraction form
+ * (every limb in one int8 contraction over 72-bit fractions of each column's
+ * CRT value, pairs near a rounding change recomputed exactly; keys of two
+ * digits that are the same integers in every limb), 1 = int8 per limb,
+ * 0 = fp64.  exact_pairs (optional): the pairs recomputed exactly.  Same key
+ * values in every form.  CUFE_CONTRACT=limbs turns the fraction form off. */
+int cufe_ctx_contract_form(const cufe_ctx *ctx, unsigned long long *exact_pairs);
+/* Number of block shapes of the fraction-form contraction; CUFE_FRAC_VARIANT=v
+ * (0 <= v < this) forces shape v instead of the autotuned one. */
+int cufe_contract_frac_variants(void);
 /* Reduction interval of the fp64 contraction and the largest balanced
  * functional-key coefficient, per limb. */
 int cufe_ctx_model_info(const cufe_ctx *ctx, int *kb, unsigned int *maxabs);

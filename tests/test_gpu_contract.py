@@ -1,9 +1,10 @@
-"""The coefficient-0 contraction kernels agree bit for bit: the int8 matrix-core
-kernel in every block shape (and its autotuned choice), with the CRT and
-rounding fused into the last limb's epilogue or run as a separate kernel, the
-fp64 scalar-operand kernel and the fp64 LDS kernel, on one model and one row
-store.  Keys too large
-for the two-digit split run on fp64 and still match the oracle."""
+"""The coefficient-0 contraction kernels agree bit for bit: the fraction form
+(every limb in one int8 contraction, in every block shape), the per-limb int8
+matrix-core kernel in every block shape (and its autotuned choice), with the
+CRT and rounding fused into the last limb's epilogue or run as a separate
+kernel, the fp64 scalar-operand kernel and the fp64 LDS kernel, on one model
+and one row store.  Keys too large for the two-digit split run on fp64 and
+still match the oracle."""
 import hashlib
 import os
 
@@ -69,11 +70,13 @@ def test_kernels_agree(gpu, hi, digits):
     variants = gpu.contract_variants(digits)
     note(f"variants {variants}")
     assert len(variants) >= 6
-    # every variant (the fused CRT epilogue on limb 2), the separate CRT
-    # kernel, and the fp64 kernels
-    for env in [{"CUFE_I8_VARIANT": str(v)} for v in variants] + [{"CUFE_NO_FUSED_CRT": "1"},
-                                                                  {"CUFE_CONTRACT": "sgpr"},
-                                                                  {"CUFE_CONTRACT": "lds"}]:
+    # every per-limb variant (the fused CRT epilogue on limb 2), the separate
+    # CRT kernel, the fp64 kernels and every fraction-form variant (two key
+    # digits only)
+    limbs = [{"CUFE_I8_VARIANT": str(v), "CUFE_CONTRACT": "limbs"} for v in variants]
+    fracs = [{"CUFE_FRAC_VARIANT": str(v)} for v in range(gpu.frac_variants())] if digits == 2 else []
+    for env in limbs + fracs + [{"CUFE_NO_FUSED_CRT": "1", "CUFE_CONTRACT": "limbs"}, {"CUFE_CONTRACT": "sgpr"},
+                                {"CUFE_CONTRACT": "lds"}]:
         note(f"kernel {env}")
         lab, kv = predict_with(ctx, R, env)
         assert np.array_equal(kv, kv0), env
@@ -105,7 +108,8 @@ def test_longest_rows_every_variant(gpu):
     lab0, kv0 = predict_with(ctx, R, {})
     ip = (x.astype(np.int64) @ sv.astype(np.int64).T) % 50241
     assert np.array_equal(kv0.astype(np.int64) % 50241, ip)
-    for env in [{"CUFE_I8_VARIANT": str(v)} for v in gpu.contract_variants(2)] + [{"CUFE_CONTRACT": "sgpr"}]:
+    for env in ([{"CUFE_I8_VARIANT": str(v), "CUFE_CONTRACT": "limbs"} for v in gpu.contract_variants(2)] +
+                [{"CUFE_FRAC_VARIANT": str(v)} for v in range(gpu.frac_variants())] + [{"CUFE_CONTRACT": "sgpr"}]):
         lab, kv = predict_with(ctx, R, env)
         assert np.array_equal(kv, kv0), env
         assert np.array_equal(lab, lab0), env

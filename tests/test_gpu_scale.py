@@ -101,7 +101,7 @@ def test_row_ranges_equal_whole(scale):
 def test_separate_crt_and_fp64_agree(scale):
     ctx, _, _, _, lab, kv = scale
     note("separate CRT kernel")
-    lab_s, kv_s = predict_env(ctx, 0, R, {"CUFE_NO_FUSED_CRT": "1"})
+    lab_s, kv_s = predict_env(ctx, 0, R, {"CUFE_NO_FUSED_CRT": "1", "CUFE_CONTRACT": "limbs"})
     assert np.array_equal(kv_s, kv) and np.array_equal(lab_s, lab)
     note("fp64 contraction, rows [0, 2048)")
     lab_f, kv_f = predict_env(ctx, 0, 2048, {"CUFE_CONTRACT": "sgpr"})

@@ -40,4 +40,18 @@ __device__ __forceinline__ void tile_rc(int t, int ntk, int ntr, int gr, int kg,
     tk = sb * kg + j / rg;
 }
 
+// The same with the key tile fastest: within a group of gr row tiles, sub-
+// blocks of kg key tiles run the key tiles of one row tile consecutively, so
+// neighbouring blocks share the row tile (the heavier operand when rows carry
+// more digit planes than keys) and blocks kg apart share key tiles.
+__device__ __forceinline__ void tile_kr(int t, int ntk, int ntr, int gr, int kg, int &tr, int &tk)
+{
+    const int g = t / (gr * ntk), i = t - g * gr * ntk;
+    const int rg = min(gr, ntr - g * gr);  // row tiles of group g
+    const int sb = i / (rg * kg), j = i - sb * rg * kg;
+    const int kw = min(kg, ntk - sb * kg);  // key tiles of sub-block sb
+    tk = sb * kg + j % kw;
+    tr = g * gr + j / kw;
+}
+
 }  // namespace ipfe

@@ -24,13 +24,15 @@
 //
 // The epilogue forms V = sum_c C_c 2^(8 (8 - c)) mod 2^72 in 128-bit integers,
 // so V / 2^72 = sum_k alpha~[k] B[k] mod 1 exactly, and since
-// |alpha~ - alpha| < 2^-72.99, the true x / Q lies within M_v 2^-72 of it,
-// M_v = 1 + sum_k |B_v[k]|.  The key value floor((p V + 2^71) / 2^72) is
-// round(p x / Q) unless that interval holds a point where the rounding changes:
+// |alpha~ - alpha| <= 2^-73 + 2^-96.4 (rounding to 72 bits; the 128-bit
+// reciprocals), the true x / Q lies within M_v 2^-72 of it,
+// M_v = ceil(s / 2) + ceil(s / 2^20) + 1 with s = sum_k |B_v[k]|.  The key
+// value floor((p V + 2^71) / 2^72) is round(p x / Q) unless that interval
+// holds a point where the rounding changes:
 //   * a half-integer of p x / Q: |(p V mod 2^72) - 2^71| <= p M_v;
 //   * the wrap of x / Q from 1 to 0, where the reference's value jumps from p
 //     to 0 (V < M_v or V > 2^72 - M_v): a zero inner product whose noise is
-//     below about 2^37.
+//     below about 2^36.
 // Such a pair is listed, and k_frac_exact computes its key value from the
 // limbs' residues exactly (64-bit sums of the balanced row and key values).
 // At a wrap only the sign of x' (x or x - Q, |x'| < 2 M_v Q 2^-72 < 2^44 for
@@ -132,7 +134,8 @@ __global__ void k_rows_frac(const uint32_t *__restrict__ cl, const uint32_t *__r
 }
 
 // Keys: whether every key coefficient is the same integer in the three limbs
-// (kd [3][S][K] balanced; *neq set otherwise), M_v = 1 + sum_k |B_v[k]|,
+// (kd [3][S][K] balanced; *neq set otherwise), M_v = ceil(s / 2) +
+// ceil(s / 2^20) + 1 with s = sum_k |B_v[k]|,
 // times 2^widen up to 2^62 (widen > 0 only to test the exact kernel on many
 // pairs), and limb 0's values as int16 k16 [S][K] for the exact kernel (two
 // key digits: |B| <= 32,512).
@@ -157,7 +160,8 @@ __global__ void k_keys_frac(const double *__restrict__ kd, int S, int K, int wid
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const unsigned long long m = part[0] + 1, cap = 1ull << 62;
+        const unsigned long long s = part[0], cap = 1ull << 62;
+        const unsigned long long m = (s + 1) / 2 + (s + (1ull << 20) - 1) / (1ull << 20) + 1;
         margin[v] = m > (cap >> widen) ? cap : m << widen;
     }
 }
@@ -178,7 +182,8 @@ template <int WR, int WK, int STAGES, int OCC>
 __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *__restrict__ A,
                                                                    const v4i *__restrict__ B, int R, int S,
                                                                    int ksteps, long long a_stride, long long b_stride,
-                                                                   FracOut out, int ntk, int ntiles, int gr, int kg)
+                                                                   FracOut out, int ntk, int ntiles, int gr, int kg,
+                                                                   int key_fast)
 {
     constexpr int W = WR * WK;
     constexpr int FA = WR * FRAC_PLANES, FB = WK * 2;
@@ -195,7 +200,10 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
     const int t = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
     if (t >= ntiles) return;
     int tr, tk;
-    tile_rc(t, ntk, ntiles / ntk, gr, kg, tr, tk);
+    if (key_fast)
+        tile_kr(t, ntk, ntiles / ntk, gr, kg, tr, tk);
+    else
+        tile_rc(t, ntk, ntiles / ntk, gr, kg, tr, tk);
     const v4i *base[FW];
     long long step[FW];
 #pragma unroll
@@ -474,13 +482,18 @@ int launch_keys_frac(const double *kd, int S, int K, int widen, uint64_t *margin
 }
 
 // Block tiles of the fraction contraction: waves, ring depth, waves per SIMD
-// the registers are budgeted for, tile order (gr x kg groups).
+// the registers are budgeted for, tile order (gr x kg groups, kg = 0:
+// row-major; key_fast: the key tile fastest in a group).  A row tile carries
+// 9 digit planes per column, a key tile 2, so the orders that keep a row
+// tile's blocks together fetch the least (the row-fastest 8 x 8 groups of
+// 64 x 64 tiles fetched twice the bytes of row-major order and ran slowest).
 struct FracVariant {
-    int wr, wk, stages, occ, gr, kg;
+    int wr, wk, stages, occ, gr, kg, key_fast;
 };
 static const FracVariant kFracVariants[] = {
-    {2, 2, 3, 2, 8, 8}, {2, 2, 3, 2, 1, 0}, {1, 4, 3, 2, 16, 4}, {1, 4, 4, 2, 16, 4}, {2, 4, 3, 2, 8, 4},
-    {4, 2, 3, 2, 4, 8},
+    {2, 2, 3, 2, 1, 0, 0},  {1, 4, 3, 2, 16, 4, 0}, {1, 4, 4, 2, 16, 4, 0}, {2, 4, 3, 2, 8, 4, 0},
+    {4, 2, 3, 2, 4, 8, 0},  {4, 2, 3, 2, 1, 0, 0},  {2, 2, 3, 2, 2, 32, 1}, {1, 4, 3, 2, 4, 16, 1},
+    {2, 4, 3, 2, 2, 16, 1}, {4, 2, 3, 2, 2, 16, 1},
 };
 constexpr int kNumFracVariants = sizeof(kFracVariants) / sizeof(kFracVariants[0]);
 
@@ -490,7 +503,8 @@ void frac_shape(int v, int *wr, int *wk)
     *wr = kFracVariants[v].wr;
     *wk = kFracVariants[v].wk;
 }
-// "fWRxWK/STAGES" plus "gGRxKG" for the grouped tile order
+// "fWRxWK/STAGES" plus "gGRxKG" for the grouped tile order ("kGRxKG" with
+// the key tile fastest)
 const char *frac_variant_label(int v)
 {
     struct Labels {
@@ -500,7 +514,8 @@ const char *frac_variant_label(int v)
             for (int i = 0; i < kNumFracVariants; ++i) {
                 const FracVariant &x = kFracVariants[i];
                 if (x.kg)
-                    snprintf(s[i], sizeof s[i], "f%dx%d/%dg%dx%d", x.wr, x.wk, x.stages, x.gr, x.kg);
+                    snprintf(s[i], sizeof s[i], "f%dx%d/%d%c%dx%d", x.wr, x.wk, x.stages, x.key_fast ? 'k' : 'g', x.gr,
+                             x.kg);
                 else
                     snprintf(s[i], sizeof s[i], "f%dx%d/%d", x.wr, x.wk, x.stages);
             }
@@ -530,15 +545,17 @@ int launch_contract_frac(const int8_t *A, int ksteps_a, const int8_t *B, int kst
     const long long as = (long long)ksteps_a * FRAC_PLANES * FRAG_V4, bs = (long long)ksteps * 2 * FRAG_V4;
 #define CFR(wr, wk, stg, occ)                                                                                      \
     hipLaunchKernelGGL((k_contract_frac<wr, wk, stg, occ>), grid, block, 0, st, a, b, R, S, ksteps, as, bs, out, ntk, \
-                       ntiles, gr, kg)
-    switch (variant) {
-    case 0:
-    case 1: CFR(2, 2, 3, 2); break;
-    case 2: CFR(1, 4, 3, 2); break;
-    case 3: CFR(1, 4, 4, 2); break;
-    case 4: CFR(2, 4, 3, 2); break;
-    default: CFR(4, 2, 3, 2); break;  // variant 5
-    }
+                       ntiles, gr, kg, x.key_fast)
+    if (x.wr == 2 && x.wk == 2)
+        CFR(2, 2, 3, 2);
+    else if (x.wr == 1 && x.stages == 3)
+        CFR(1, 4, 3, 2);
+    else if (x.wr == 1)
+        CFR(1, 4, 4, 2);
+    else if (x.wr == 2)
+        CFR(2, 4, 3, 2);
+    else
+        CFR(4, 2, 3, 2);
 #undef CFR
     HIP_OK(hipGetLastError());
     return 0;

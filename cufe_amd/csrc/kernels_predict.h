@@ -60,7 +60,7 @@ struct FracConsts {
 };
 struct FracOut {
     uint32_t *kval;          // [R][S] key values
-    const uint64_t *margin;  // [S]: 1 + sum_k |B_v[k]|
+    const uint64_t *margin;  // [S]: error bound of each key's sum, units of 2^-72
     uint32_t *amb;           // [amb_cap][2]: (row, key) of the pairs left to the exact kernel
     uint32_t *amb_count;     // zeroed before the launch
     int amb_cap;
@@ -71,7 +71,8 @@ size_t frac_digit_bytes(int cap, int ksteps);
 int launch_rows_frac(const uint32_t *cl, const uint32_t *c0, int src_l, int R, int r0, int cap, int ksteps,
                      int8_t *dig, hipStream_t st);
 // keys: *neq set unless every coefficient is the same integer (within int16)
-// in all limbs; margin[v] = (1 + sum_k |kd[0][v][k]|) 2^widen, at most 2^62
+// in all limbs; margin[v] = the bound M_v on |x / Q - V 2^-72| in units of
+// 2^-72 (M_v ~ sum_k |kd[0][v][k]| / 2) times 2^widen, at most 2^62
 // (widen <= 62; > 0 only in tests, to send many pairs to the exact kernel);
 // k16 [S][K] = limb 0's values
 int launch_keys_frac(const double *kd, int S, int K, int widen, uint64_t *margin, int16_t *k16, int *neq,

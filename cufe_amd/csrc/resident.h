@@ -36,7 +36,7 @@ struct Model {
     DevBuf<int8_t> kdig;
     // the fraction-form contraction applies (two key digits, fused keys, every
     // key coefficient the same integer in the three limbs); margin [S] holds
-    // 1 + sum_k |B_v[k]| of each key
+    // each key's error bound (about sum_k |B_v[k]| / 2, units of 2^-72)
     bool frac = false;
     DevBuf<uint64_t> margin;
     DevBuf<int16_t> k16;  // [S][kkey] key values for the fraction form's exact kernel

@@ -1,0 +1,98 @@
+"""The arithmetic of the fraction-form contraction (k_contract_frac.hip), on
+the CPU against the oracle: for real ciphertexts and functional keys, the
+72-bit fixed-point sum V = 2^72 sum_k alpha~[k] B[k] mod 2^72 built from nine
+signed digits per column lies within the kernel's margin M_v of
+2^72 x / Q (x = CRT of the limbs' exact residues), and floor((p V + 2^71) /
+2^72) equals the oracle's rounding wherever the margin test passes; the
+dropped digit pair (1, 1) contributes only multiples of 2^72."""
+import hashlib
+from fractions import Fraction
+
+import numpy as np
+
+import conftest  # noqa: F401  (oracle path)
+import pyoracle as po
+
+QS = [16760833, 2147352577, 2130706433]
+Q = QS[0] * QS[1] * QS[2]
+P = 50241
+N = 4096
+
+
+def h(t: bytes) -> bytes:
+    return hashlib.sha256(t).digest()
+
+
+def bal(w, q):
+    w = int(w) % q
+    return w - q if w > q // 2 else w
+
+
+def alpha_digits(words):
+    """The kernel's k_rows_frac: alpha = sum_j (w_j u_j mod q_j) floor(2^128 / q_j)
+    mod 2^128, rounded to 72 bits, as nine digits (a_2..a_9 balanced, a_1 mod 256)."""
+    acc = 0
+    for j, q in enumerate(QS):
+        u = pow(Q // q % q, q - 2, q)
+        w = int(words[j])
+        w = w - q if w >= q else w
+        acc = (acc + (w * u % q) * (((1 << 128) - 1) // q)) % (1 << 128)
+    t = ((acc + (1 << 55)) % (1 << 128)) >> 56
+    a = [0] * 9
+    for d in range(8, 0, -1):
+        v = ((t & 255) + 128) % 256 - 128
+        a[d] = v
+        t = (t - v) >> 8
+    a[0] = (t & 255) - 256 if (t & 255) >= 128 else (t & 255)
+    return a
+
+
+def test_fraction_sum_within_margin_and_rounds_like_the_oracle():
+    l = 24
+    mpk, msk = po.setup(h(b"mA"), h(b"mS"), h(b"mE"), l)
+    rng = np.random.default_rng(3)
+    rows = rng.integers(0, 3, (3, l)).astype(np.uint32)
+    svs = np.zeros((4, l), np.uint32)
+    for v in range(4):
+        svs[v, rng.choice(l - 1, 5, replace=False)] = rng.integers(1, 3, 5)
+    # SV 0 meets every row in zeros: inner product 0, key value 0 or p by the
+    # noise's sign (the wrap of x / Q)
+    rows[:, :5] = 0
+    svs[0] = 0
+    svs[0, 1:4] = 1
+    checked = wraps = 0
+    for r in range(rows.shape[0]):
+        c = po.encrypt(rows[r], mpk, h(b"m2"), h(b"m3"), r)
+        cols = [c[l, :, k] for k in range(N)] + [c[i, :, 0] for i in range(l)]
+        digits = [alpha_digits(w) for w in cols]
+        # with the pair (d, e) = (1, 1) left out, as the kernel does
+        hi = [a[0] for a in digits]
+        lo_all = [sum(a[d] * 256 ** (8 - d) for d in range(9)) for a in digits]
+        for v in range(svs.shape[0]):
+            sk = po.keygen(svs[v], msk)
+            key = []
+            for k in range(N):
+                vals = [bal(sk[j, 0] if k == 0 else (QS[j] - sk[j, N - k]) % QS[j], QS[j]) for j in range(3)]
+                assert vals[0] == vals[1] == vals[2]
+                key.append(-vals[0])
+            key += [int(y) for y in svs[v]]
+            b1 = [(b - (((b + 128) & 255) - 128)) // 256 for b in key]
+            V = sum(x * b for x, b in zip(lo_all, key)) % (1 << 72)
+            V17 = (V - sum(256 ** 9 * a1 * e1 for a1, e1 in zip(hi, b1))) % (1 << 72)
+            assert V17 == V  # the (1, 1) products are multiples of 2^72
+            d = [sum(bal(cols[k][j], QS[j]) * key[k] for k in range(len(key))) % QS[j] for j in range(3)]
+            x = po.crt_value(*d)
+            s = sum(abs(b) for b in key)
+            M = (s + 1) // 2 + (s + (1 << 20) - 1) // (1 << 20) + 1
+            T = Fraction(x * (1 << 72), Q)
+            err = abs(T - V)
+            err = min(err, (1 << 72) - err)
+            assert err < M
+            pv = P * V
+            ambiguous = V < M or V > (1 << 72) - M or abs(pv % (1 << 72) - (1 << 71)) <= P * M
+            if not ambiguous:
+                kv = (pv + (1 << 71)) >> 72
+                assert kv == po.crt_round(*d)
+                checked += 1
+                wraps += kv in (0, P)
+    assert checked >= 10 and wraps >= 3

@@ -825,7 +825,11 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
     if (!dev) return -1;
     hipStream_t st = dev->stream;
     // rows per chunk: D (3 words) + kvalue (1 word) per (row, key) within ~16 GiB
-    const int chunk_rows = (int)std::max<size_t>(64, ((size_t)1 << 30) / (size_t)c->model.S);
+    // (CUFE_PREDICT_CHUNK_ROWS=n, for tests: smaller chunks, so that a call
+    // of a few thousand rows runs several)
+    const char *cr = getenv("CUFE_PREDICT_CHUNK_ROWS");
+    const int chunk_rows = cr && atoi(cr) >= 64 ? atoi(cr)
+                                                : (int)std::max<size_t>(64, ((size_t)1 << 30) / (size_t)c->model.S);
     PredictTimers tm;
     for (int i = 0; i < 4; ++i) tm.ev[i] = c->ev[i];
     tm.ev[4] = c->ev[8];

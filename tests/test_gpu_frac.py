@@ -116,6 +116,29 @@ def test_exact_kernel_on_many_pairs(gpu, widen):
     ctx.close()
 
 
+def test_chunked_calls_equal_one_chunk(gpu):
+    """A call whose rows run in several chunks (each its own contraction, list
+    and exact kernel) gives the one-chunk call's results; with margins wide
+    enough that each chunk lists pairs, the totals add up."""
+    rng = np.random.default_rng(45)
+    l, R, S = 124, 1836, 150  # chunks of 512: 512 x 3 + 300 (every chunk above the split-K size)
+    ctx, x = rows_ctx(gpu, rng, R, l, b"c")
+    with Env(CUFE_FRAC_WIDEN="26"):
+        ctx.set_model(binary_model(gpu, rng, S, l))
+    lab, kv, _ = ctx.predict(0, R, want_kvalues=True)
+    form, exact = ctx.contract_form()
+    assert form == 2 and exact > 0
+    with Env(CUFE_PREDICT_CHUNK_ROWS="512"):
+        lab_c, kv_c, _ = ctx.predict(0, R, want_kvalues=True)
+        form_c, exact_c = ctx.contract_form()
+    note(f"one chunk: {exact} listed; chunks of 512 rows: {exact_c}")
+    assert form_c == 2 and exact_c == exact
+    assert np.array_equal(kv_c, kv) and np.array_equal(lab_c, lab)
+    lab_l, kv_l = per_limb(ctx, 0, R)
+    assert np.array_equal(kv, kv_l) and np.array_equal(lab, lab_l)
+    ctx.close()
+
+
 def test_list_overflow_runs_per_limb(gpu):
     """Every pair near a rounding change and more of them than the exact
     kernel's list holds (2^20): the call runs again per limb, same results."""

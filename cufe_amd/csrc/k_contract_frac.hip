@@ -169,9 +169,9 @@ __global__ void k_keys_frac(const double *__restrict__ kd, int S, int K, int wid
 // ---------------------------------------------------------------------------
 // The contraction.  A block of WR x WK waves computes a (32 WR) x (32 WK)
 // tile, one 32 x 32 block per wave with nine accumulators.  Per k step the
-// block streams WR x 9 row and WK x 2 key fragments (plus padding loads that
-// make the count a multiple of the waves, re-reading key fragment 0 into a
-// spare slot) from global memory into an LDS ring of STAGES steps
+// block streams WR x 9 row and WK x 2 key fragments (the first FR waves one
+// load more than the others; each wave's wait counts its own loads) from
+// global memory into an LDS ring of STAGES steps
 // (global_load_lds_dwordx4), as in k_contract_i8_il: the 17 MFMAs of a step
 // are interleaved by hand with the wait and barrier for the next slot (after
 // MFMA 1), the next ring loads (after MFMA 2), the next step's key fragments
@@ -187,9 +187,9 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
 {
     constexpr int W = WR * WK;
     constexpr int FA = WR * FRAC_PLANES, FB = WK * 2;
-    constexpr int F = (FA + FB + W - 1) / W * W;  // fragments per step with the padding loads
-    constexpr int FW = F / W;
-    constexpr int STEP_V4 = F * FRAG_V4;
+    // fragments per step: waves w < FR load FW of them, the others FW - 1
+    constexpr int FW = (FA + FB + W - 1) / W, FR = FA + FB - (FW - 1) * W;
+    constexpr int STEP_V4 = (FA + FB) * FRAG_V4;
     static_assert(STAGES >= 3, "the main loop waits for the step after next");
     __shared__ v4i ring[STAGES * STEP_V4];
     const int lane = threadIdx.x & 63;
@@ -199,6 +199,7 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
     // contiguous share of the tiles
     const int t = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
     if (t >= ntiles) return;
+    const bool wfull = w < FR;  // this wave loads FW fragments per step
     int tr, tk;
     if (key_fast)
         tile_kr(t, ntk, ntiles / ntk, gr, kg, tr, tk);
@@ -217,8 +218,8 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
             base[u] = B + (size_t)(tk * WK + fb / 2) * b_stride + (fb % 2) * FRAG_V4 + lane;
             step[u] = 2 * FRAG_V4;
         } else {
-            base[u] = B + (size_t)(tk * WK) * b_stride + lane;  // padding: key fragment 0 again
-            step[u] = 2 * FRAG_V4;
+            base[u] = B;  // a wave's last slot past the step's fragments: never loaded
+            step[u] = 0;
         }
     }
     const int G = ksteps;
@@ -228,8 +229,9 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
 #pragma unroll
         for (int u = 0; u < FW; ++u) {
             const int f = w + u * W;
-            __builtin_amdgcn_global_load_lds((glb_void_t *)(base[u] + (size_t)it_ks * step[u]),
-                                             (lds_void_t *)(slot + f * FRAG_V4), 16, 0, 0);
+            if (u < FW - 1 || wfull)
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(base[u] + (size_t)it_ks * step[u]),
+                                                 (lds_void_t *)(slot + f * FRAG_V4), 16, 0, 0);
         }
         it_slot = it_slot + 1 == STAGES ? 0 : it_slot + 1;
         ++it_ks;
@@ -242,10 +244,12 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
 #pragma unroll
     for (int p = 0; p < STAGES; ++p)
         if (p < G) issue_next();
-    if (G >= STAGES)
+    if (G < STAGES)
+        wait_vmcnt<0>();
+    else if (wfull)
         wait_vmcnt<(STAGES - 1) * FW>();
     else
-        wait_vmcnt<0>();
+        wait_vmcnt<(STAGES - 1) * (FW - 1)>();
     asm volatile("s_barrier" ::: "memory");
     v4i fa[FRAC_PLANES], fb0[2], fb1[2];
 #pragma unroll
@@ -265,10 +269,12 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
             acc[c] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[frac_plane(i)], fb[frac_kdig(i)], acc[c], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
             if (i == 1) {
-                if constexpr (M)
+                if constexpr (!M)
+                    wait_vmcnt<0>();
+                else if (wfull)
                     wait_vmcnt<(STAGES - 2) * FW>();
                 else
-                    wait_vmcnt<0>();
+                    wait_vmcnt<(STAGES - 2) * (FW - 1)>();
                 asm volatile("s_barrier" ::: "memory");
             } else if (i == 2) {
                 if constexpr (M)

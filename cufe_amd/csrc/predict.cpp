@@ -458,6 +458,7 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
     // the flag is raised only by the separate key-value fold (unfused keys)
     if (!m.fused && launch_fill_i32(sc.flag.get(), 0, 1, st)) return -1;
     const bool frac_ok = mode >= 3 && m.i8 && m.frac && rs.Fdig.get() && !getenv("CUFE_NO_FRAC");
+    sc.frac_chunks = 0;
     if (frac_ok) {
         if ((!sc.amb.get() && sc.amb.alloc((size_t)2 * kAmbCap)) || (!sc.amb_count.get() && sc.amb_count.alloc(1)) ||
             (!sc.amb_overflow.get() && sc.amb_overflow.alloc(1)) || (!sc.amb_total.get() && sc.amb_total.alloc(1)))
@@ -494,6 +495,7 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
         // the listed pairs exactly
         const bool frac = !split && i8 && frac_ok;
         if (frac) {
+            ++sc.frac_chunks;
             const FracOut fo = {sc.kval.get(), m.margin.get(), sc.amb.get(), sc.amb_count.get(), kAmbCap};
             if (contract_frac_tuned(rs.Fdig.get() + (size_t)(r0 / 32) * rs.ksteps * 9 * 1024, rs.ksteps, m.kdig.get(),
                                     m.ksteps, R, S, fo, st))
@@ -859,7 +861,7 @@ extern "C" int cufe_ctx_predict(cufe_ctx *c, int row0, int nrows, double *labels
         HIP_OK(hipEventRecord(c->ev[5], st));
         HIP_OK(hipEventSynchronize(c->ev[5]));
     }
-    c->form = c->model.i8 && mode >= 2 ? (frac && !overflow ? 2 : 1) : 0;
+    c->form = c->model.i8 && mode >= 2 ? (c->scratch.frac_chunks > 0 ? 2 : 1) : 0;
     c->exact_pairs = exact_pairs;
     if (tm.pending && predict_timers_collect(&tm)) return -1;
     float tot = 0;

@@ -83,6 +83,7 @@ struct Scratch {
     DevBuf<uint32_t> amb, amb_count;
     DevBuf<int> amb_overflow;
     DevBuf<unsigned long long> amb_total;
+    int frac_chunks = 0;  // chunks of the last predict_rows call that ran the fraction form
     // small row ranges may use the split-K launch (a server turns it off while
     // its other slot runs: there it would slow that slot's contraction)
     bool split_ok = true;

@@ -257,10 +257,11 @@ int cufe_ctx_predict(cufe_ctx *ctx, int row0, int nrows, double *labels, uint32_
  * ms[1] = its exact kernel, ms[2] = 0. */
 int cufe_ctx_contract_limb_ms(const cufe_ctx *ctx, float *ms);
 /* Form of the last cufe_ctx_predict call's contraction: 2 = the fraction form
- * (every limb in one int8 contraction over 72-bit fractions of each column's
- * CRT value, pairs near a rounding change recomputed exactly; keys of two
- * digits that are the same integers in every limb), 1 = int8 per limb,
- * 0 = fp64.  exact_pairs (optional): the pairs recomputed exactly.  Same key
+ * ran on at least one row chunk (every limb in one int8 contraction over
+ * 72-bit fractions of each column's CRT value, pairs near a rounding change
+ * recomputed exactly; keys of two digits that are the same integers in every
+ * limb; chunks of at most 256 rows run the split-K per-limb launch instead),
+ * 1 = int8 per limb, 0 = fp64.  exact_pairs (optional): the pairs recomputed exactly.  Same key
  * values in every form.  CUFE_CONTRACT=limbs turns the fraction form off. */
 int cufe_ctx_contract_form(const cufe_ctx *ctx, unsigned long long *exact_pairs);
 /* Number of block shapes of the fraction-form contraction; CUFE_FRAC_VARIANT=v

@@ -493,13 +493,17 @@ int launch_keys_frac(const double *kd, int S, int K, int widen, uint64_t *margin
 // 9 digit planes per column, a key tile 2, so the orders that keep a row
 // tile's blocks together fetch the least (the row-fastest 8 x 8 groups of
 // 64 x 64 tiles fetched twice the bytes of row-major order and ran slowest).
+// An XCD runs 32 of the 8-wave blocks at a time on consecutive tiles: for
+// 64 x 128 tiles, sub-blocks of 4 row x 8 key tiles balance the row digits
+// re-read per sub-block of key tiles against the key digits re-read per
+// sub-block of row tiles, and fetch the least of that shape.
 struct FracVariant {
     int wr, wk, stages, occ, gr, kg, key_fast;
 };
 static const FracVariant kFracVariants[] = {
     {2, 2, 3, 2, 1, 0, 0},  {1, 4, 3, 2, 16, 4, 0}, {1, 4, 4, 2, 16, 4, 0}, {2, 4, 3, 2, 8, 4, 0},
     {4, 2, 3, 2, 4, 8, 0},  {4, 2, 3, 2, 1, 0, 0},  {2, 2, 3, 2, 2, 32, 1}, {1, 4, 3, 2, 4, 16, 1},
-    {2, 4, 3, 2, 2, 16, 1}, {4, 2, 3, 2, 2, 16, 1},
+    {2, 4, 3, 2, 2, 16, 1}, {4, 2, 3, 2, 2, 16, 1}, {2, 4, 3, 2, 4, 8, 1}, {2, 4, 3, 2, 4, 8, 0},
 };
 constexpr int kNumFracVariants = sizeof(kFracVariants) / sizeof(kFracVariants[0]);
 

@@ -347,6 +347,30 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
 // chunk's first row; k16: the key values [S][K] (the same integers in every
 // limb).  A count above the list's capacity sets *overflow (those pairs' key
 // values are then missing).
+//
+// Each lane takes groups of four columns (16-byte row loads, 8-byte key
+// loads; K, ldr and the limb stride are multiples of 4), eight groups a lane
+// in flight.  The listed pairs rarely share a row, so a pair reads its row's
+// 34 KB (two limbs) or 51 KB (three) from HBM: the kernel is bound by those
+// bytes, not by the round trips.
+template <int LIMBS>
+__device__ __forceinline__ void frac_exact_sums(const int4 *__restrict__ a0, const int4 *__restrict__ a1,
+                                                const int4 *__restrict__ a2, const int2 *__restrict__ b, int K4,
+                                                int lane, long long &s0, long long &s1, long long &s2)
+{
+    auto dot4 = [](const int4 v, const int2 w) {
+        return (long long)v.x * (int)(short)w.x + (long long)v.y * (w.x >> 16) + (long long)v.z * (int)(short)w.y +
+               (long long)v.w * (w.y >> 16);
+    };
+#pragma unroll 8
+    for (int g = lane; g < K4; g += 64) {
+        const int2 w = b[g];
+        s1 += dot4(a1[g], w);
+        s2 += dot4(a2[g], w);
+        if (LIMBS == 3) s0 += dot4(a0[g], w);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_frac_exact(const int32_t *__restrict__ A, size_t limb_stride, int ldr,
                                                     const int16_t *__restrict__ k16, int S, int K,
                                                     const uint64_t *__restrict__ margin, uint32_t inv12,
@@ -368,36 +392,16 @@ __global__ __launch_bounds__(256) void k_frac_exact(const int32_t *__restrict__ 
         const uint32_t kw = amb[2 * (size_t)i + 1];
         const int key = (int)(kw & 0x7fffffffu);
         const bool two = !(kw >> 31) && margin[key] < (1ull << 44);
-        const int16_t *b = k16 + (size_t)key * K;
-        const int32_t *a0 = A + (size_t)row * ldr, *a1 = a0 + limb_stride, *a2 = a1 + limb_stride;
+        const int2 *b = reinterpret_cast<const int2 *>(k16 + (size_t)key * K);
+        const int32_t *r0p = A + (size_t)row * ldr;
+        const int4 *a0 = reinterpret_cast<const int4 *>(r0p);
+        const int4 *a1 = reinterpret_cast<const int4 *>(r0p + limb_stride);
+        const int4 *a2 = reinterpret_cast<const int4 *>(r0p + 2 * limb_stride);
         long long s0 = 0, s1 = 0, s2 = 0;
-        // four columns per lane in flight
-        int k = lane;
-        for (; k + 192 < K; k += 256) {
-            long long bv[4];
-            int v1[4], v2[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                bv[u] = b[k + 64 * u];
-                v1[u] = a1[k + 64 * u];
-                v2[u] = a2[k + 64 * u];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                s1 += (long long)v1[u] * bv[u];
-                s2 += (long long)v2[u] * bv[u];
-            }
-            if (!two) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) s0 += (long long)a0[k + 64 * u] * bv[u];
-            }
-        }
-        for (; k < K; k += 64) {
-            const long long bv = b[k];
-            s1 += (long long)a1[k] * bv;
-            s2 += (long long)a2[k] * bv;
-            if (!two) s0 += (long long)a0[k] * bv;
-        }
+        if (two)
+            frac_exact_sums<2>(a0, a1, a2, b, K >> 2, lane, s0, s1, s2);
+        else
+            frac_exact_sums<3>(a0, a1, a2, b, K >> 2, lane, s0, s1, s2);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             s0 += __shfl_xor(s0, o);
@@ -575,6 +579,12 @@ int launch_frac_exact(const int32_t *A, size_t limb_stride, int ldr, const int16
                       const FracOut &out, int *overflow, unsigned long long *total, hipStream_t st)
 {
     static const uint32_t inv12 = powmod(qmod(1) % qmod(2), qmod(2) - 2, qmod(2));
+    // 16-byte row and 8-byte key loads: every row, limb and key starts on a
+    // multiple of 4 elements
+    if (K % 4 || ldr % 4 || limb_stride % 4 || K > ldr || ((uintptr_t)A & 15) || ((uintptr_t)k16 & 7)) {
+        ipfe_set_error("fraction exact kernel: %d key columns, row stride %d, limb stride %zu", K, ldr, limb_stride);
+        return -1;
+    }
     hipLaunchKernelGGL(k_frac_exact, dim3(2048), dim3(256), 0, st, A, limb_stride, ldr, k16, S, K, out.margin, inv12,
                        out.amb, out.amb_count, out.amb_cap, out.kval, overflow, total);
     HIP_OK(hipGetLastError());

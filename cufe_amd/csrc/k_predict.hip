@@ -388,19 +388,34 @@ __global__ __launch_bounds__(256) void k_decide_wave(const uint32_t *__restrict_
     if (EXACT_INT && m.nr_class == 2 && (S & 3) == 0) {
         // Two classes: SV t (of either class) has coefficient sv_coef[t], so
         // the decision value is one dot product over the row; each lane loads
-        // 4 consecutive key values (16 B) and their 4 coefficients (32 B).
+        // 4 consecutive key values (16 B) and their 4 coefficients (16 B as
+        // int32, summed in 64-bit integers, else 32 B of doubles).
         const uint4 *kv4 = reinterpret_cast<const uint4 *>(kv);
-        const double2 *c2 = reinterpret_cast<const double2 *>(m.sv_coef);
         double part = 0.0;
+        if (m.sv_icoef) {
+            const int4 *c4 = reinterpret_cast<const int4 *>(m.sv_icoef);
+            long long acc = 0;
 #pragma unroll 4
-        for (int t = lane; t < (S >> 2); t += 64) {
-            const uint4 k = kv4[t];
-            const double2 a = c2[2 * t], b = c2[2 * t + 1];
-            part += (__dmul_rn(a.x, (double)k.x) + __dmul_rn(a.y, (double)k.y)) +
-                    (__dmul_rn(b.x, (double)k.z) + __dmul_rn(b.y, (double)k.w));
-        }
+            for (int t = lane; t < (S >> 2); t += 64) {
+                const uint4 k = kv4[t];
+                const int4 a = c4[t];
+                acc += (long long)a.x * k.x + (long long)a.y * k.y + (long long)a.z * k.z + (long long)a.w * k.w;
+            }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            part = (double)acc;  // |acc| < 2^53: exact
+        } else {
+            const double2 *c2 = reinterpret_cast<const double2 *>(m.sv_coef);
+#pragma unroll 4
+            for (int t = lane; t < (S >> 2); t += 64) {
+                const uint4 k = kv4[t];
+                const double2 a = c2[2 * t], b = c2[2 * t + 1];
+                part += (__dmul_rn(a.x, (double)k.x) + __dmul_rn(a.y, (double)k.y)) +
+                        (__dmul_rn(b.x, (double)k.z) + __dmul_rn(b.y, (double)k.w));
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        }
         if (lane == 0) labels[r] = (double)m.label[__dsub_rn(part, m.rho[0]) > 0 ? 0 : 1];
         return;
     }

@@ -15,6 +15,7 @@ struct SvmModelDev {
     const double *sv_coef;  // [nr_class-1][S]
     const double *rho;      // [nr_class*(nr_class-1)/2]
     int exact_int;          // integer coefficients, sum |coef| * p < 2^53: sums exact in any order
+    const int *sv_icoef;    // exact_int with every |coef| < 2^31: the coefficients as int32 (else null)
 };
 
 int launch_contract_keys(const uint32_t *sk, int S, int K, int sign, const SparseY &y, double *kd, unsigned int *maxabs,

@@ -265,6 +265,19 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
             abs_sum += fabs(c);
         }
         md.exact_int = integral && abs_sum * 50241.0 < 9007199254740992.0;
+        // the two-class vote then reads 4-byte coefficients (half the bytes of
+        // the doubles) and sums in 64-bit integers: the same exact sum
+        icoef.reset();
+        bool fits = md.exact_int != 0;
+        for (int k = 0; fits && k < (m->nr_class - 1) * nS; ++k) fits = fabs(m->sv_coef[k]) < 2147483648.0;
+        if (fits) {
+            std::vector<int32_t> h((size_t)(m->nr_class - 1) * nS);
+            for (size_t k = 0; k < h.size(); ++k) h[k] = (int32_t)m->sv_coef[k];
+            if (icoef.alloc(h.size())) return -1;
+            HIP_OK(hipMemcpyAsync(icoef.get(), h.data(), h.size() * 4, hipMemcpyHostToDevice, st));
+            if (sync_stream(st)) return -1;
+            md.sv_icoef = icoef.get();
+        }
     }
     mdev = md;
     // reduction interval (16-product steps) of the fp64 contraction: every
@@ -314,7 +327,7 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
 
 size_t Model::bytes() const
 {
-    return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8) + kdig.n + margin.n * 8 + k16.n * 2;
+    return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8) + kdig.n + margin.n * 8 + k16.n * 2 + icoef.n * 4;
 }
 
 // ---------------------------------------------------------------------------

@@ -28,6 +28,7 @@ struct Model {
     DevBuf<uint32_t> sk;  // [S][3][N]
     DevBuf<double> kd;    // [3][S][kkey]
     DevBuf<double> coef, rho;
+    DevBuf<int32_t> icoef;  // integer coefficients for the exact two-class vote (else empty)
     SvmModelDev mdev{};
     // int8-digit key store [3][spad/32][ksteps][nb][64][16] when every key
     // coefficient fits nb = 2 or 3 signed digits (nb = 0: fp64 kernels only)

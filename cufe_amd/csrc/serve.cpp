@@ -100,9 +100,11 @@ struct cufe_server {
     size_t budget = 0;
     int max_batch = 0;
     int batching = 1;
-    // worker's poll interval while a batch is in flight (us; 0 = spin);
+    // worker's poll interval while a batch is in flight (us; 0 = query the
+    // events continuously, yielding the core between queries: a batch is
+    // retired as soon as it ends, at the cost of a core while the GPU works);
     // CUFE_SERVE_POLL_US overrides
-    int poll_us = 50;
+    int poll_us = 0;
     std::vector<std::unique_ptr<Model>> models;
     size_t model_bytes = 0;
     int S_max = 0, l_max = 0, cap = 0;

@@ -23,20 +23,32 @@
 // C_c, c = d - e - 1 in 0..8: |C_c| <= 2 K 2^14 < 2^31 for K <= 2^15.
 //
 // The epilogue forms V = sum_c C_c 2^(8 (8 - c)) mod 2^72 in 128-bit integers,
-// so V / 2^72 = sum_k alpha~[k] B[k] mod 1 exactly, and since
-// |alpha~ - alpha| <= 2^-73 + 2^-96.4 (rounding to 72 bits; the 128-bit
-// reciprocals), the true x / Q lies within M_v 2^-72 of it,
-// M_v = ceil(s / 2) + ceil(s / 2^20) + 1 with s = sum_k |B_v[k]|.  The key
+// so V / 2^72 = sum_k alpha~[k] B[k] mod 1 exactly, and the true x / Q lies
+// within M_v 2^-72 of it, M_v = ceil(s / 2) + ceil(s / 2^16) + 1 with
+// s = sum_k |B_v[k]| (the rounding of alpha~, below).  The key
 // value floor((p V + 2^71) / 2^72) is round(p x / Q) unless that interval
 // holds a point where the rounding changes:
 //   * a half-integer of p x / Q: |(p V mod 2^72) - 2^71| <= p M_v;
 //   * the wrap of x / Q from 1 to 0, where the reference's value jumps from p
 //     to 0 (V < M_v or V > 2^72 - M_v): a zero inner product whose noise is
 //     below about 2^36.
-// Such a pair is listed, and k_frac_exact computes its key value from the
-// limbs' residues exactly (64-bit sums of the balanced row and key values).
-// At a wrap only the sign of x' (x or x - Q, |x'| < 2 M_v Q 2^-72 < 2^44 for
-// M_v < 2^29) is open, and limbs 1 and 2 fix it: x' mod q_1 q_2 (~2^62).
+// Such a pair is listed with its V, and k_frac_exact first refines it to 88
+// bits: each row also keeps the 16 bits of alpha below the nine digits as one
+// int16 per column, D[k] = alpha~88[k] - 2^16 alpha~72[k] in [-2^15, 2^15)
+// (alpha~72 is alpha~88 rounded to 72 bits), so
+//     V88 = 2^16 V + sum_k D[k] B[k]   (mod 2^88)
+// is 2^88 sum_k alpha~88 B mod 1, within M88_v = ceil(s / 2) + ceil(s / 2^7) + 1
+// of 2^88 x / Q, and the same two tests at 88 bits settle all but the pairs
+// whose x / Q lies within about 2^-66 of a rounding change (at a wrap, noise
+// below about 2^20).  Those few are computed from the limbs' residues exactly
+// (64-bit sums of the balanced row and key values).  At a wrap only the sign
+// of x' (x or x - Q, |x'| <= 2 M88_v Q 2^-88 < 2^60 for M88_v < 2^61) is open,
+// and limbs 1 and 2 fix it: x' mod q_1 q_2 (~2^62).
+//
+// The margins: alpha itself is known to 3 2^-97 (below), alpha~88 rounds it
+// to the nearest 2^-88 and alpha~72 rounds alpha~88 to the nearest 2^-72, so
+// |alpha~72 - alpha| <= 2^-73 + 2^-89 + 3 2^-97 and M_v = ceil(s / 2) +
+// ceil(s / 2^16) + 1 bounds s times that in units of 2^-72.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 
@@ -85,17 +97,20 @@ static_assert(frac_order_ok(), "each (plane, key digit) pair once, (0, 1) left o
 // c_0..c_{src_l-1} from c0 [R][src_l][3]; words in [0, q], q taken as 0),
 // alpha = sum_j r_j / q_j mod 1 with r_j = w_j u_j mod q_j, formed as
 // sum_j r_j floor(2^128 / q_j) mod 2^128 (error below 3 2^-97), rounded to
-// 72 bits and split into nine digits.  Store: [(cap + 128) / 32 row blocks]
-// [ksteps][9 planes][64 lanes][16 B], the MFMA fragment layout.
+// 88 bits (alpha~88), that rounded to 72 bits (alpha~72) and split into nine
+// digits.  Store: [(cap + 128) / 32 row blocks][ksteps][9 planes][64 lanes]
+// [16 B], the MFMA fragment layout; the difference D = alpha~88 -
+// 2^16 alpha~72 in [-2^15, 2^15) to ref [cap][ksteps x 32] (int16, row-major).
 // ---------------------------------------------------------------------------
 __global__ void k_rows_frac(const uint32_t *__restrict__ cl, const uint32_t *__restrict__ c0, int src_l, int R, int r0,
-                            int ksteps, FracConsts fc, int8_t *__restrict__ dig)
+                            int ksteps, FracConsts fc, int8_t *__restrict__ dig, int16_t *__restrict__ ref)
 {
     const int quads = ksteps * 8;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)R * quads) return;
     const int rr = (int)(i / quads), row = r0 + rr, k = 4 * (int)(i % quads);
     int8_t a[4][FRAC_PLANES];
+    int16_t dref[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int col = k + u;
@@ -112,7 +127,9 @@ __global__ void k_rows_frac(const uint32_t *__restrict__ cl, const uint32_t *__r
             const uint32_t r = (uint32_t)(((uint64_t)w * fc.u[j]) % q);
             acc += (u128)r * (((u128)fc.rhi[j] << 64) | fc.rlo[j]);
         }
-        u128 t = (acc + ((u128)1 << 55)) >> 56;  // alpha in units of 2^-72
+        const u128 t88 = (acc + ((u128)1 << 39)) >> 40;  // alpha~88, units of 2^-88
+        u128 t = (t88 + ((u128)1 << 15)) >> 16;          // alpha~72 (2^72 taken as 0 below)
+        dref[u] = (int16_t)(int)(__int128)(t88 - (t << 16));
 #pragma unroll
         for (int d = FRAC_PLANES - 1; d >= 1; --d) {
             const int v = (((int)(t & 255) + 128) & 255) - 128;
@@ -131,16 +148,19 @@ __global__ void k_rows_frac(const uint32_t *__restrict__ cl, const uint32_t *__r
         for (int u = 0; u < 4; ++u) word |= (uint32_t)(uint8_t)a[u][p] << (8 * u);
         *reinterpret_cast<uint32_t *>(base + (size_t)p * 1024) = word;
     }
+    const uint64_t dw = (uint64_t)(uint16_t)dref[0] | ((uint64_t)(uint16_t)dref[1] << 16) |
+                        ((uint64_t)(uint16_t)dref[2] << 32) | ((uint64_t)(uint16_t)dref[3] << 48);
+    *reinterpret_cast<uint64_t *>(ref + (size_t)row * (ksteps * 32) + k) = dw;
 }
 
 // Keys: whether every key coefficient is the same integer in the three limbs
-// (kd [3][S][K] balanced; *neq set otherwise), M_v = ceil(s / 2) +
-// ceil(s / 2^20) + 1 with s = sum_k |B_v[k]|,
-// times 2^widen up to 2^62 (widen > 0 only to test the exact kernel on many
-// pairs), and limb 0's values as int16 k16 [S][K] for the exact kernel (two
-// key digits: |B| <= 32,512).
+// (kd [3][S][K] balanced; *neq set otherwise), the margins M_v = ceil(s / 2) +
+// ceil(s / 2^16) + 1 (72 bits) and M88_v = ceil(s / 2) + ceil(s / 2^7) + 1
+// (88 bits) with s = sum_k |B_v[k]|, each times 2^widen up to 2^62 (widen > 0
+// only to test the exact kernel on many pairs), and limb 0's values as int16
+// k16 [S][K] for the exact kernel (two key digits: |B| <= 32,512).
 __global__ void k_keys_frac(const double *__restrict__ kd, int S, int K, int widen, uint64_t *__restrict__ margin,
-                            int16_t *__restrict__ k16, int *__restrict__ neq)
+                            uint64_t *__restrict__ margin88, int16_t *__restrict__ k16, int *__restrict__ neq)
 {
     __shared__ unsigned long long part[256];
     const int v = blockIdx.x;
@@ -161,8 +181,10 @@ __global__ void k_keys_frac(const double *__restrict__ kd, int S, int K, int wid
     }
     if (threadIdx.x == 0) {
         const unsigned long long s = part[0], cap = 1ull << 62;
-        const unsigned long long m = (s + 1) / 2 + (s + (1ull << 20) - 1) / (1ull << 20) + 1;
+        const unsigned long long m = (s + 1) / 2 + (s + (1ull << 16) - 1) / (1ull << 16) + 1;
+        const unsigned long long m88 = (s + 1) / 2 + (s + 127) / 128 + 1;
         margin[v] = m > (cap >> widen) ? cap : m << widen;
+        margin88[v] = m88 > (cap >> widen) ? cap : m88 << widen;
     }
 }
 
@@ -328,31 +350,39 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
         if (!wrap && !halfway) {
             out.kval[(size_t)row * S + key] = (uint32_t)((pv + half) >> 72);
         } else {
-            // bit 31 of the key word: near a half-integer (else only the wrap)
+            // bit 31 of the key word: near a half-integer (else only the wrap);
+            // then V's 72 bits for the refinement
             const uint32_t n = atomicAdd(out.amb_count, 1u);
             if ((int)n < out.amb_cap) {
-                out.amb[2 * (size_t)n] = (uint32_t)row;
-                out.amb[2 * (size_t)n + 1] = (uint32_t)key | (halfway ? 0x80000000u : 0u);
+                uint32_t *e = out.amb + (size_t)FRAC_AMB_WORDS * n;
+                e[0] = (uint32_t)row;
+                e[1] = (uint32_t)key | (halfway ? 0x80000000u : 0u);
+                e[2] = (uint32_t)V;
+                e[3] = (uint32_t)(V >> 32);
+                e[4] = (uint32_t)(V >> 64);
             }
         }
     }
 }
 
-// The listed pairs exactly: one wave per pair sums A_j[r][k] B_v[k] over k in
-// 64-bit integers (|A| < 2^30, |B| < 2^15, K < 2^13: below 2^58) and reduces
-// mod q_j.  A pair listed only for the wrap (bit 31 of its key word clear)
-// whose margin is below 2^44 needs limbs 1 and 2: x' = the centred value of
+// The listed pairs.  One wave per pair.  With refine set, it first forms
+// V88 = 2^16 V + sum_k D[r][k] B_v[k] mod 2^88 (ref: the rows' D values, row
+// stride kf; 16-byte loads of eight columns) and writes the key value when
+// neither test fires at 88 bits.  Otherwise -- and for every pair without
+// refine, a switch for tests -- it sums A_j[r][k] B_v[k] over k in 64-bit
+// integers (|A| < 2^30, |B| < 2^15, K < 2^13: below 2^58) and reduces mod
+// q_j.  A pair open only at the wrap whose margin allows it (M88_v < 2^61, or
+// without refine M_v < 2^44) needs limbs 1 and 2: x' = the centred value of
 // x mod q_1 q_2 (Garner), key value 0 for x' >= 0 and p below; the others
 // decode all three limbs.  A: the balanced row store [3][cap][ldr] at the
-// chunk's first row; k16: the key values [S][K] (the same integers in every
-// limb).  A count above the list's capacity sets *overflow (those pairs' key
-// values are then missing).
+// chunk's first row, ref the D store at the same row; k16: the key values
+// [S][K] (the same integers in every limb).  A count above the list's
+// capacity sets *overflow (those pairs' key values are then missing).
 //
-// Each lane takes groups of four columns (16-byte row loads, 8-byte key
-// loads; K, ldr and the limb stride are multiples of 4), eight groups a lane
-// in flight.  The listed pairs rarely share a row, so a pair reads its row's
-// 34 KB (two limbs) or 51 KB (three) from HBM: the kernel is bound by those
-// bytes, not by the round trips.
+// Each lane takes groups of four (exact sums) or eight (refinement) columns,
+// eight groups a lane in flight; K, kf, ldr and the limb stride are multiples
+// of 8.  The listed pairs rarely share a row, so the exact sums read a row's
+// 34 KB (two limbs) or 51 KB (three) from HBM, the refinement its 8.4 KB of D.
 template <int LIMBS>
 __device__ __forceinline__ void frac_exact_sums(const int4 *__restrict__ a0, const int4 *__restrict__ a1,
                                                 const int4 *__restrict__ a2, const int2 *__restrict__ b, int K4,
@@ -371,9 +401,18 @@ __device__ __forceinline__ void frac_exact_sums(const int4 *__restrict__ a0, con
     }
 }
 
+// sum of eight int16 products of two 16-byte groups (|sum| < 2^33)
+__device__ __forceinline__ long long dot8_i16(const int4 x, const int4 y)
+{
+    auto two = [](int u, int v) { return (long long)((int)(short)u * (int)(short)v) + (long long)((u >> 16) * (v >> 16)); };
+    return two(x.x, y.x) + two(x.y, y.y) + two(x.z, y.z) + two(x.w, y.w);
+}
+
 __global__ __launch_bounds__(256) void k_frac_exact(const int32_t *__restrict__ A, size_t limb_stride, int ldr,
+                                                    const int16_t *__restrict__ ref, int kf,
                                                     const int16_t *__restrict__ k16, int S, int K,
-                                                    const uint64_t *__restrict__ margin, uint32_t inv12,
+                                                    const uint64_t *__restrict__ margin,
+                                                    const uint64_t *__restrict__ margin88, int refine, uint32_t inv12,
                                                     const uint32_t *__restrict__ amb,
                                                     const uint32_t *__restrict__ amb_count, int amb_cap,
                                                     uint32_t *__restrict__ kval, int *__restrict__ overflow,
@@ -388,10 +427,35 @@ __global__ __launch_bounds__(256) void k_frac_exact(const int32_t *__restrict__ 
     const int lane = threadIdx.x & 63;
     const int nw = (int)(gridDim.x * blockDim.x / 64);
     for (int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / 64); i < n; i += nw) {
-        const int row = (int)amb[2 * (size_t)i];
-        const uint32_t kw = amb[2 * (size_t)i + 1];
+        const uint32_t *e = amb + (size_t)FRAC_AMB_WORDS * i;
+        const int row = (int)e[0];
+        const uint32_t kw = e[1];
         const int key = (int)(kw & 0x7fffffffu);
-        const bool two = !(kw >> 31) && margin[key] < (1ull << 44);
+        bool two;
+        if (refine) {
+            const int4 *d8 = reinterpret_cast<const int4 *>(ref + (size_t)row * kf);
+            const int4 *b8 = reinterpret_cast<const int4 *>(k16 + (size_t)key * K);
+            long long r = 0;
+#pragma unroll 8
+            for (int g = lane; g < (K >> 3); g += 64) r += dot8_i16(d8[g], b8[g]);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o);
+            // every lane holds the sum: the tests below are the same on all
+            const u128 mask88 = ((u128)1 << 88) - 1, half = (u128)1 << 87, one = (u128)1 << 88;
+            const u128 V = (u128)e[2] | ((u128)e[3] << 32) | ((u128)(e[4] & 255u) << 64);
+            const u128 V88 = ((V << 16) + (u128)(__int128)r) & mask88;
+            const uint64_t mv = margin88[key];
+            const u128 pv = (u128)IPFE_P * V88, fr = pv & mask88, pm = (u128)IPFE_P * mv;
+            const bool wrap = V88 < mv || V88 > one - mv;
+            const bool halfway = fr + pm >= half && fr <= half + pm;
+            if (!wrap && !halfway) {
+                if (lane == 0) kval[(size_t)row * S + key] = (uint32_t)((pv + half) >> 88);
+                continue;
+            }
+            two = !halfway && mv < (1ull << 61);
+        } else {
+            two = !(kw >> 31) && margin[key] < (1ull << 44);
+        }
         const int2 *b = reinterpret_cast<const int2 *>(k16 + (size_t)key * K);
         const int32_t *r0p = A + (size_t)row * ldr;
         const int4 *a0 = reinterpret_cast<const int4 *>(r0p);
@@ -463,7 +527,7 @@ FracConsts frac_consts()
 size_t frac_digit_bytes(int cap, int ksteps) { return (size_t)(cap + 128) / 32 * ksteps * FRAC_PLANES * 1024; }
 
 int launch_rows_frac(const uint32_t *cl, const uint32_t *c0, int src_l, int R, int r0, int cap, int ksteps,
-                     int8_t *dig, hipStream_t st)
+                     int8_t *dig, int16_t *ref, hipStream_t st)
 {
     const size_t n = (size_t)R * ksteps * 8;
     if (n == 0) return 0;
@@ -473,20 +537,21 @@ int launch_rows_frac(const uint32_t *cl, const uint32_t *c0, int src_l, int R, i
     }
     static const FracConsts fc = frac_consts();
     hipLaunchKernelGGL(k_rows_frac, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cl, c0, src_l, R, r0, ksteps,
-                       fc, dig);
+                       fc, dig, ref);
     HIP_OK(hipGetLastError());
     return 0;
 }
 
-int launch_keys_frac(const double *kd, int S, int K, int widen, uint64_t *margin, int16_t *k16, int *neq,
-                     hipStream_t st)
+int launch_keys_frac(const double *kd, int S, int K, int widen, uint64_t *margin, uint64_t *margin88, int16_t *k16,
+                     int *neq, hipStream_t st)
 {
     if (S <= 0) return 0;
     if (widen < 0 || widen > 62) {
         ipfe_set_error("fraction keys: widening 2^%d out of range", widen);
         return -1;
     }
-    hipLaunchKernelGGL(k_keys_frac, dim3((unsigned)S), dim3(256), 0, st, kd, S, K, widen, margin, k16, neq);
+    hipLaunchKernelGGL(k_keys_frac, dim3((unsigned)S), dim3(256), 0, st, kd, S, K, widen, margin, margin88, k16,
+                       neq);
     HIP_OK(hipGetLastError());
     return 0;
 }
@@ -575,18 +640,22 @@ int launch_contract_frac(const int8_t *A, int ksteps_a, const int8_t *B, int kst
     return 0;
 }
 
-int launch_frac_exact(const int32_t *A, size_t limb_stride, int ldr, const int16_t *k16, int S, int K,
-                      const FracOut &out, int *overflow, unsigned long long *total, hipStream_t st)
+int launch_frac_exact(const int32_t *A, size_t limb_stride, int ldr, const int16_t *ref, int kf, bool refine,
+                      const int16_t *k16, int S, int K, const FracOut &out, int *overflow, unsigned long long *total,
+                      hipStream_t st)
 {
     static const uint32_t inv12 = powmod(qmod(1) % qmod(2), qmod(2) - 2, qmod(2));
-    // 16-byte row and 8-byte key loads: every row, limb and key starts on a
-    // multiple of 4 elements
-    if (K % 4 || ldr % 4 || limb_stride % 4 || K > ldr || ((uintptr_t)A & 15) || ((uintptr_t)k16 & 7)) {
-        ipfe_set_error("fraction exact kernel: %d key columns, row stride %d, limb stride %zu", K, ldr, limb_stride);
+    // 16-byte loads: every row, limb and key starts on a multiple of 8
+    // elements, in 16-byte aligned stores
+    if (K % 8 || ldr % 8 || kf % 8 || limb_stride % 8 || K > ldr || K > kf || ((uintptr_t)A & 15) ||
+        ((uintptr_t)k16 & 15) || ((uintptr_t)ref & 15)) {
+        ipfe_set_error("fraction exact kernel: %d key columns, row strides %d and %d, limb stride %zu", K, ldr, kf,
+                       limb_stride);
         return -1;
     }
-    hipLaunchKernelGGL(k_frac_exact, dim3(2048), dim3(256), 0, st, A, limb_stride, ldr, k16, S, K, out.margin, inv12,
-                       out.amb, out.amb_count, out.amb_cap, out.kval, overflow, total);
+    hipLaunchKernelGGL(k_frac_exact, dim3(2048), dim3(256), 0, st, A, limb_stride, ldr, ref, kf, k16, S, K, out.margin,
+                       out.margin88, refine ? 1 : 0, inv12, out.amb, out.amb_count, out.amb_cap, out.kval, overflow,
+                       total);
     HIP_OK(hipGetLastError());
     return 0;
 }

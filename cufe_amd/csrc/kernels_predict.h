@@ -59,34 +59,42 @@ struct FracConsts {
     uint32_t u[3];            // (Q / q_j)^-1 mod q_j
     uint64_t rlo[3], rhi[3];  // floor(2^128 / q_j)
 };
+// words of a listed pair: row, key (bit 31: near a half-integer), V's 72 bits
+constexpr int FRAC_AMB_WORDS = 5;
 struct FracOut {
-    uint32_t *kval;          // [R][S] key values
-    const uint64_t *margin;  // [S]: error bound of each key's sum, units of 2^-72
-    uint32_t *amb;           // [amb_cap][2]: (row, key) of the pairs left to the exact kernel
-    uint32_t *amb_count;     // zeroed before the launch
+    uint32_t *kval;            // [R][S] key values
+    const uint64_t *margin;    // [S]: error bound of each key's sum, units of 2^-72
+    const uint64_t *margin88;  // [S]: the same at 88 bits, units of 2^-88
+    uint32_t *amb;             // [amb_cap][FRAC_AMB_WORDS]: the pairs left to the exact kernel
+    uint32_t *amb_count;       // zeroed before the launch
     int amb_cap;
 };
 FracConsts frac_consts();
 // bytes of the fraction digit store [(cap + 128) / 32][ksteps][9][64][16]
 size_t frac_digit_bytes(int cap, int ksteps);
+// digits of rows [r0, r0 + R) into dig, and their 16 bits below the digits
+// (int16 D) into ref [cap][ksteps x 32]
 int launch_rows_frac(const uint32_t *cl, const uint32_t *c0, int src_l, int R, int r0, int cap, int ksteps,
-                     int8_t *dig, hipStream_t st);
+                     int8_t *dig, int16_t *ref, hipStream_t st);
 // keys: *neq set unless every coefficient is the same integer (within int16)
 // in all limbs; margin[v] = the bound M_v on |x / Q - V 2^-72| in units of
-// 2^-72 (M_v ~ sum_k |kd[0][v][k]| / 2) times 2^widen, at most 2^62
-// (widen <= 62; > 0 only in tests, to send many pairs to the exact kernel);
-// k16 [S][K] = limb 0's values
-int launch_keys_frac(const double *kd, int S, int K, int widen, uint64_t *margin, int16_t *k16, int *neq,
-                     hipStream_t st);
+// 2^-72 (M_v ~ sum_k |kd[0][v][k]| / 2), margin88[v] the same at 88 bits,
+// both times 2^widen, at most 2^62 (widen <= 62; > 0 only in tests, to send
+// many pairs to the exact kernel); k16 [S][K] = limb 0's values
+int launch_keys_frac(const double *kd, int S, int K, int widen, uint64_t *margin, uint64_t *margin88, int16_t *k16,
+                     int *neq, hipStream_t st);
 // A: fraction digits at the first row (a multiple of 32), B: limb 0's
 // two-digit key store
 int launch_contract_frac(const int8_t *A, int ksteps_a, const int8_t *B, int ksteps, int R, int S, const FracOut &out,
                          hipStream_t st, int variant);
-// the listed pairs from the balanced row store (A at the first row, limb j at
-// + j limb_stride) and the key values k16 [S][K]; *overflow set when the list
-// overflowed; *total (if given) += the listed pairs
-int launch_frac_exact(const int32_t *A, size_t limb_stride, int ldr, const int16_t *k16, int S, int K,
-                      const FracOut &out, int *overflow, unsigned long long *total, hipStream_t st);
+// the listed pairs: refined to 88 bits from ref (the rows' D values at the
+// first row, row stride kf) unless refine is false, the rest from the
+// balanced row store (A at the first row, limb j at + j limb_stride), with the
+// key values k16 [S][K]; *overflow set when the list overflowed; *total (if
+// given) += the listed pairs
+int launch_frac_exact(const int32_t *A, size_t limb_stride, int ldr, const int16_t *ref, int kf, bool refine,
+                      const int16_t *k16, int S, int K, const FracOut &out, int *overflow, unsigned long long *total,
+                      hipStream_t st);
 int frac_variant_count();
 void frac_shape(int variant, int *wr, int *wk);
 const char *frac_variant_label(int variant);

@@ -294,6 +294,7 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
     i8 = nb > 0;
     kdig.reset();
     margin.reset();
+    margin88.reset();
     k16.reset();
     frac = false;
     if (i8) {
@@ -309,8 +310,10 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
         DevBuf<int> neq;
         int h_neq = 1;
         const char *wv = getenv("CUFE_FRAC_WIDEN");
-        if (margin.alloc(nS) || k16.alloc((size_t)nS * kk) || neq.alloc(1) || launch_fill_i32(neq.get(), 0, 1, st) ||
-            launch_keys_frac(kd.get(), nS, kk, wv ? atoi(wv) : 0, margin.get(), k16.get(), neq.get(), st))
+        if (margin.alloc(nS) || margin88.alloc(nS) || k16.alloc((size_t)nS * kk) || neq.alloc(1) ||
+            launch_fill_i32(neq.get(), 0, 1, st) ||
+            launch_keys_frac(kd.get(), nS, kk, wv ? atoi(wv) : 0, margin.get(), margin88.get(), k16.get(), neq.get(),
+                             st))
             return -1;
         HIP_OK(hipMemcpyAsync(&h_neq, neq.get(), 4, hipMemcpyDeviceToHost, st));
         if (sync_stream(st)) return -1;
@@ -327,7 +330,8 @@ int Model::build(const uint32_t *msk_dev, int L, const cufe_svm_model *m, hipStr
 
 size_t Model::bytes() const
 {
-    return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8) + kdig.n + margin.n * 8 + k16.n * 2 + icoef.n * 4;
+    return (size_t)S * (kPoly * 4 + 3 * (size_t)kkey * 8) + kdig.n + (margin.n + margin88.n) * 8 + k16.n * 2 +
+           icoef.n * 4;
 }
 
 // ---------------------------------------------------------------------------
@@ -337,7 +341,7 @@ size_t RowStore::bytes_per_row(int l, bool fractions)
 {
     const size_t ldr = IPFE_N + ((l + 15) / 16) * 16;
     const size_t kd = IPFE_N + ((l + 31) / 32) * 32;
-    return 3 * ldr * (4 + 8) + 3 * kd * 4 + (size_t)l * 12 + (fractions ? 9 * kd : 0);
+    return 3 * ldr * (4 + 8) + 3 * kd * 4 + (size_t)l * 12 + (fractions ? (9 + 2) * kd : 0);
 }
 
 int RowStore::reserve(int cap_rows, int len, bool fractions)
@@ -355,9 +359,11 @@ int RowStore::reserve(int cap_rows, int len, bool fractions)
     HIP_OK(hipMemset(A.get(), 0, (size_t)3 * cp * ld * 4));
     HIP_OK(hipMemset(Adig.get(), 0, Adig.n));
     Fdig.reset();
+    Fref.reset();
     if (fractions) {
-        if (Fdig.alloc(frac_digit_bytes(cp, ks))) return -1;
+        if (Fdig.alloc(frac_digit_bytes(cp, ks)) || Fref.alloc((size_t)cp * ks * 32)) return -1;
         HIP_OK(hipMemset(Fdig.get(), 0, Fdig.n));
+        HIP_OK(hipMemset(Fref.get(), 0, Fref.n * 2));
     }
     a64_ok.assign((size_t)cp / 64, 0);
     cap = cp;
@@ -377,7 +383,8 @@ int RowStore::put(int r0, int n, const uint32_t *cl_dev, const uint32_t *c0_dev,
     if (n == 0) return 0;
     if (launch_rows_put(cl_dev, c0_dev, src_l, n, r0, cap, ldr, l, ksteps, A.get(), c0.get(), Adig.get(), st))
         return -1;
-    if (Fdig.get() && launch_rows_frac(cl_dev, c0_dev, src_l, n, r0, cap, ksteps, Fdig.get(), st)) return -1;
+    if (Fdig.get() && launch_rows_frac(cl_dev, c0_dev, src_l, n, r0, cap, ksteps, Fdig.get(), Fref.get(), st))
+        return -1;
     for (int b = r0 / 64; b <= (r0 + n - 1) / 64; ++b) a64_ok[b] = 0;
     return 0;
 }
@@ -408,7 +415,7 @@ int RowStore::ensure_a64(int r0, int R, hipStream_t st) const
 // under load the split launch slows the other slot's contraction more than it
 // gains.
 constexpr int kSplitMaxRows = 256;
-// pairs the fraction form can leave to its exact kernel per chunk (8 MiB of
+// pairs the fraction form can leave to its exact kernel per chunk (20 MiB of
 // list); a chunk with more sets the overflow flag and the caller runs the
 // prediction again per limb
 constexpr int kAmbCap = 1 << 20;
@@ -473,7 +480,8 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
     const bool frac_ok = mode >= 3 && m.i8 && m.frac && rs.Fdig.get() && !getenv("CUFE_NO_FRAC");
     sc.frac_chunks = 0;
     if (frac_ok) {
-        if ((!sc.amb.get() && sc.amb.alloc((size_t)2 * kAmbCap)) || (!sc.amb_count.get() && sc.amb_count.alloc(1)) ||
+        if ((!sc.amb.get() && sc.amb.alloc((size_t)FRAC_AMB_WORDS * kAmbCap)) ||
+            (!sc.amb_count.get() && sc.amb_count.alloc(1)) ||
             (!sc.amb_overflow.get() && sc.amb_overflow.alloc(1)) || (!sc.amb_total.get() && sc.amb_total.alloc(1)))
             return -1;
         if (launch_fill_i32(sc.amb_overflow.get(), 0, 1, st) || launch_fill_i32((int *)sc.amb_total.get(), 0, 2, st))
@@ -509,12 +517,16 @@ int predict_rows(const Model &m, const RowStore &rs, int row0, int nrows, Scratc
         const bool frac = !split && i8 && frac_ok;
         if (frac) {
             ++sc.frac_chunks;
-            const FracOut fo = {sc.kval.get(), m.margin.get(), sc.amb.get(), sc.amb_count.get(), kAmbCap};
+            const FracOut fo = {sc.kval.get(), m.margin.get(), m.margin88.get(), sc.amb.get(), sc.amb_count.get(),
+                                kAmbCap};
             if (contract_frac_tuned(rs.Fdig.get() + (size_t)(r0 / 32) * rs.ksteps * 9 * 1024, rs.ksteps, m.kdig.get(),
                                     m.ksteps, R, S, fo, st))
                 return -1;
             if (tm) HIP_OK(hipEventRecord(tm->ev[4], st));
-            if (launch_frac_exact(rs.A.get() + (size_t)r0 * rs.ldr, (size_t)rs.cap * rs.ldr, rs.ldr, m.k16.get(), S,
+            // CUFE_FRAC_NO_REFINE=1 (tests): every listed pair from the limbs' residues
+            const int kf = rs.ksteps * 32;
+            if (launch_frac_exact(rs.A.get() + (size_t)r0 * rs.ldr, (size_t)rs.cap * rs.ldr, rs.ldr,
+                                  rs.Fref.get() + (size_t)r0 * kf, kf, !getenv("CUFE_FRAC_NO_REFINE"), m.k16.get(), S,
                                   m.kkey, fo, sc.amb_overflow.get(), sc.amb_total.get(), st))
                 return -1;
             if (tm) HIP_OK(hipEventRecord(tm->ev[5], st));

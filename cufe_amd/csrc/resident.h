@@ -39,7 +39,7 @@ struct Model {
     // key coefficient the same integer in the three limbs); margin [S] holds
     // each key's error bound (about sum_k |B_v[k]| / 2, units of 2^-72)
     bool frac = false;
-    DevBuf<uint64_t> margin;
+    DevBuf<uint64_t> margin, margin88;  // the same bound at 88 bits (units of 2^-88) for the refinement
     DevBuf<int16_t> k16;  // [S][kkey] key values for the fraction form's exact kernel
 
     int build(const uint32_t *msk_dev, int l_key, const cufe_svm_model *m, hipStream_t st);
@@ -52,13 +52,16 @@ struct Model {
 // fp64), as int8 digit fragments for the matrix-core contraction
 // ([3][(cap+128)/32][ksteps][4][64][16], ksteps = (N + l rounded up to 32) / 32),
 // optionally as the nine fraction digits of each column's CRT value
-// ([(cap+128)/32][ksteps][9][64][16], for the fraction-form contraction), and
-// [cap][l][3] canonical coefficient-0 residues.
+// ([(cap+128)/32][ksteps][9][64][16], for the fraction-form contraction) with
+// the next 16 bits of each as one int16 ([cap][ksteps x 32], for the 88-bit
+// refinement of pairs near a rounding change), and [cap][l][3] canonical
+// coefficient-0 residues.
 struct RowStore {
     int l = 0, cap = 0, ldr = 0, ksteps = 0;
     DevBuf<int32_t> A;
     DevBuf<int8_t> Adig;
-    DevBuf<int8_t> Fdig;  // empty unless reserved with fractions
+    DevBuf<int8_t> Fdig;   // empty unless reserved with fractions
+    DevBuf<int16_t> Fref;  // likewise
     DevBuf<uint32_t> c0;
     // fp64 blocked copy, built on demand (only the fp64 fallback kernel reads
     // it): allocated at first use, converted per 64-row block when stale

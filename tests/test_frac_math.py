@@ -4,7 +4,10 @@ the CPU against the oracle: for real ciphertexts and functional keys, the
 signed digits per column lies within the kernel's margin M_v of
 2^72 x / Q (x = CRT of the limbs' exact residues), and floor((p V + 2^71) /
 2^72) equals the oracle's rounding wherever the margin test passes; the
-dropped digit pair (1, 1) contributes only multiples of 2^72."""
+dropped digit pair (1, 1) contributes only multiples of 2^72.  The exact
+kernel's refinement V88 = 2^16 V + sum_k D[k] B[k] mod 2^88 (D: the 16 bits
+below the digits) lies within M88_v of 2^88 x / Q and decides every pair the
+same way at 88 bits."""
 import hashlib
 from fractions import Fraction
 
@@ -30,21 +33,35 @@ def bal(w, q):
 
 def alpha_digits(words):
     """The kernel's k_rows_frac: alpha = sum_j (w_j u_j mod q_j) floor(2^128 / q_j)
-    mod 2^128, rounded to 72 bits, as nine digits (a_2..a_9 balanced, a_1 mod 256)."""
+    mod 2^128, rounded to 88 bits, that rounded to 72 bits, as nine digits
+    (a_2..a_9 balanced, a_1 mod 256), and D = alpha~88 - 2^16 alpha~72."""
     acc = 0
     for j, q in enumerate(QS):
         u = pow(Q // q % q, q - 2, q)
         w = int(words[j])
         w = w - q if w >= q else w
         acc = (acc + (w * u % q) * (((1 << 128) - 1) // q)) % (1 << 128)
-    t = ((acc + (1 << 55)) % (1 << 128)) >> 56
+    t88 = ((acc + (1 << 39)) % (1 << 128)) >> 40
+    t = (t88 + (1 << 15)) >> 16
+    dref = t88 - (t << 16)
+    assert -(1 << 15) <= dref < (1 << 15)
     a = [0] * 9
     for d in range(8, 0, -1):
         v = ((t & 255) + 128) % 256 - 128
         a[d] = v
         t = (t - v) >> 8
     a[0] = (t & 255) - 256 if (t & 255) >= 128 else (t & 255)
-    return a
+    return a, dref
+
+
+def decide(V, M, bits):
+    """The kernel's tests at `bits` bits: the key value, or None near a
+    rounding change (a half-integer of p x / Q, or the wrap of x / Q)."""
+    one, half = 1 << bits, 1 << (bits - 1)
+    pv = P * V
+    if V < M or V > one - M or abs(pv % one - half) <= P * M:
+        return None
+    return (pv + half) >> bits
 
 
 def test_fraction_sum_within_margin_and_rounds_like_the_oracle():
@@ -64,7 +81,9 @@ def test_fraction_sum_within_margin_and_rounds_like_the_oracle():
     for r in range(rows.shape[0]):
         c = po.encrypt(rows[r], mpk, h(b"m2"), h(b"m3"), r)
         cols = [c[l, :, k] for k in range(N)] + [c[i, :, 0] for i in range(l)]
-        digits = [alpha_digits(w) for w in cols]
+        parts = [alpha_digits(w) for w in cols]
+        digits = [a for a, _ in parts]
+        dref = [dd for _, dd in parts]
         # with the pair (d, e) = (1, 1) left out, as the kernel does
         hi = [a[0] for a in digits]
         lo_all = [sum(a[d] * 256 ** (8 - d) for d in range(9)) for a in digits]
@@ -83,16 +102,24 @@ def test_fraction_sum_within_margin_and_rounds_like_the_oracle():
             d = [sum(bal(cols[k][j], QS[j]) * key[k] for k in range(len(key))) % QS[j] for j in range(3)]
             x = po.crt_value(*d)
             s = sum(abs(b) for b in key)
-            M = (s + 1) // 2 + (s + (1 << 20) - 1) // (1 << 20) + 1
+            M = (s + 1) // 2 + (s + (1 << 16) - 1) // (1 << 16) + 1
+            M88 = (s + 1) // 2 + (s + 127) // 128 + 1
             T = Fraction(x * (1 << 72), Q)
             err = abs(T - V)
             err = min(err, (1 << 72) - err)
             assert err < M
-            pv = P * V
-            ambiguous = V < M or V > (1 << 72) - M or abs(pv % (1 << 72) - (1 << 71)) <= P * M
-            if not ambiguous:
-                kv = (pv + (1 << 71)) >> 72
-                assert kv == po.crt_round(*d)
+            ref = po.crt_round(*d)
+            kv = decide(V, M, 72)
+            if kv is not None:
+                assert kv == ref
                 checked += 1
                 wraps += kv in (0, P)
+            V88 = (V * (1 << 16) + sum(dd * b for dd, b in zip(dref, key))) % (1 << 88)
+            T88 = Fraction(x * (1 << 88), Q)
+            err88 = abs(T88 - V88)
+            err88 = min(err88, (1 << 88) - err88)
+            assert err88 < M88
+            # a pair open at 72 bits is settled at 88 unless x / Q lies within
+            # about 2^-66 of a rounding change
+            assert decide(V88, M88, 88) == ref
     assert checked >= 10 and wraps >= 3

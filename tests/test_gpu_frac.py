@@ -1,6 +1,7 @@
 """The fraction form of the batched decryption: every limb in one int8
 contraction over 72-bit fractions of each column's CRT value, pairs near a
-rounding change recomputed exactly.  It must give the key values and labels of
+rounding change refined to 88 bits and, where that is not enough, recomputed
+exactly.  It must give the key values and labels of
 the per-limb exact contraction on every pair: with the default margins, with
 margins widened so that many pairs (including the zero inner products whose
 key value is 0 or p) go to the exact kernel, and with margins so wide that the
@@ -97,19 +98,25 @@ def test_fraction_form_equals_per_limb(gpu):
     ctx.close()
 
 
+@pytest.mark.parametrize("refine", [True, False])
 @pytest.mark.parametrize("widen", [24, 30])
-def test_exact_kernel_on_many_pairs(gpu, widen):
+def test_exact_kernel_on_many_pairs(gpu, widen, refine):
     """Margins 2^widen times wider: the zero inner products (wrap of x / Q)
     and a share of the others go to the exact kernel, which must give the
-    per-limb key values, 0 and p included."""
+    per-limb key values, 0 and p included -- through its 88-bit refinement
+    (whose widened margins still leave many zero inner products to the
+    limbs' residues), or, with the refinement switched off, every listed
+    pair from the limbs' residues (near a half-integer: all three limbs)."""
     rng = np.random.default_rng(42 + widen)
     l, R, S = 124, 600, 200
     ctx, x = rows_ctx(gpu, rng, R, l, b"w")
     with Env(CUFE_FRAC_WIDEN=str(widen)):
         ctx.set_model(binary_model(gpu, rng, S, l))
-    lab, kv, _ = ctx.predict(0, R, want_kvalues=True)
+    env = {} if refine else {"CUFE_FRAC_NO_REFINE": "1"}
+    with Env(**env):
+        lab, kv, _ = ctx.predict(0, R, want_kvalues=True)
     form, exact = ctx.contract_form()
-    note(f"widen 2^{widen}: {exact} of {R * S} pairs recomputed exactly")
+    note(f"widen 2^{widen}, refinement {refine}: {exact} of {R * S} pairs listed for the exact kernel")
     assert form == 2 and 0 < exact < R * S
     lab_l, kv_l = per_limb(ctx, 0, R)
     assert np.array_equal(kv, kv_l) and np.array_equal(lab, lab_l)

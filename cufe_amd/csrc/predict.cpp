@@ -421,6 +421,11 @@ constexpr int kSplitMaxRows = 256;
 constexpr int kAmbCap = 1 << 20;
 constexpr size_t kSplitMaxPairs = (size_t)1 << 19;
 
+bool split_k_range(const Model &m, int R)
+{
+    return m.i8 && m.fused && m.nb == 2 && R <= kSplitMaxRows && (size_t)R * m.S <= kSplitMaxPairs;
+}
+
 size_t Scratch::split_bytes(int rows, int S)
 {
     return (size_t)48 * std::min((size_t)std::min(rows, kSplitMaxRows) * S, kSplitMaxPairs) * 4;

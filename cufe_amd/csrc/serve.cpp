@@ -87,7 +87,8 @@ struct Slot {
     std::vector<int> req_row, req_group;
     std::vector<Group> groups;
     bool busy = false;
-    int batch_rows = 0;  // rows of the batch in flight
+    int batch_rows = 0;  // rows of the batch in flight (its aligned layout)
+    bool big = false;    // a group of the batch in flight runs the per-limb launches
     size_t batch_bytes = 0;  // HBM its rows occupy (rows x bytes per row)
     bool failed = false;
     std::string err;
@@ -213,13 +214,17 @@ int launch_batch(cufe_server *s, Slot &sl)
         i = j;
     }
     // small groups take the low-latency split-K launch unless another slot
-    // runs a large batch (under load the split launch would slow it more than
-    // it gains)
+    // runs a group on the per-limb launches (under load the split launch
+    // would slow that contraction more than it gains).  A batch of at most
+    // kBatchRows rows never does, whatever its aligned layout (up to 31
+    // padding rows per model): only a request part larger than that.
     sl.batch_rows = row;
     sl.batch_bytes = (size_t)row * s->row_bytes;
+    sl.big = false;
+    for (const Group &gr : sl.groups) sl.big = sl.big || !split_k_range(*gr.m, gr.R);
     bool large_other = false;
     for (const auto &o : s->slots)
-        if (o.get() != &sl && o->busy && o->batch_rows > 256) large_other = true;
+        if (o.get() != &sl && o->busy && o->big) large_other = true;
     sl.sc.split_ok = !large_other;
     for (size_t g = 0; g < sl.groups.size(); ++g) {
         const Group &gr = sl.groups[g];

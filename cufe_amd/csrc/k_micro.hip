@@ -93,8 +93,9 @@ __global__ __launch_bounds__(256) void k_mb_aes_rep(const uint32_t *__restrict__
     out[blockIdx.x * blockDim.x + threadIdx.x] = x;
 }
 
-// T0 and T2 bank-replicated (64 KiB): the sampler's default AES.
-__global__ __launch_bounds__(256) void k_mb_aes_rep2(const uint32_t *__restrict__ rk,
+// T0 and T2 bank-replicated (64 KiB): the sampler's default AES, in
+// 512-thread workgroups as the sampler runs it (two per CU).
+__global__ __launch_bounds__(512) void k_mb_aes_rep2(const uint32_t *__restrict__ rk,
                                                      const uint32_t *__restrict__ te_global,
                                                      uint32_t nblocks_per_thread, uint32_t *__restrict__ out)
 {
@@ -329,7 +330,8 @@ extern "C" int cufe_microbench(int kind, double *result)
             else if (kind == 4)
                 hipLaunchKernelGGL(k_mb_aes_rep, dim3(blocks), dim3(256), 0, st, rk.get(), g_aes_te, 64u, out.get());
             else
-                hipLaunchKernelGGL(k_mb_aes_rep2, dim3(blocks), dim3(256), 0, st, rk.get(), g_aes_te, 64u, out.get());
+                hipLaunchKernelGGL(k_mb_aes_rep2, dim3(blocks / 2), dim3(512), 0, st, rk.get(), g_aes_te, 64u,
+                                   out.get());
             HIP_OK(hipGetLastError());
             HIP_OK(hipEventRecord(b, st));
             HIP_OK(hipEventSynchronize(b));

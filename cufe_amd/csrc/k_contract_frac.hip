@@ -188,6 +188,27 @@ __global__ void k_keys_frac(const double *__restrict__ kd, int S, int K, int wid
     }
 }
 
+// V = sum_c C_c 2^(8 (8 - c)) mod 2^72 for output i of a wave's accumulators,
+// as lo + 2^64 hi: equal-width pairs first in 64-bit integers (|C_c| < 2^31,
+// so every partial sum stays below 2^57), V = q0 2^40 + q1 2^8 + C_8, whose
+// 64-bit words are the wrapped low words plus, above them, the arithmetic
+// high parts q0 >> 24, q1 >> 56, C_8 >> 63 and the two carries.
+static_assert(FRAC_ACC == 9, "the merge tree below has nine accumulators");
+__device__ __forceinline__ void frac_value(const v16i (&acc)[FRAC_ACC], int i, uint64_t &lo, uint32_t &hi)
+{
+    const long long p0 = (long long)acc[0][i] * 256 + acc[1][i];  // weight 2^56
+    const long long p1 = (long long)acc[2][i] * 256 + acc[3][i];  // 2^40
+    const long long p2 = (long long)acc[4][i] * 256 + acc[5][i];  // 2^24
+    const long long p3 = (long long)acc[6][i] * 256 + acc[7][i];  // 2^8
+    const long long q0 = p0 * 65536 + p1;                          // 2^40
+    const long long q1 = p2 * 65536 + p3;                          // 2^8
+    const long long c8 = acc[8][i];
+    const uint64_t a = (uint64_t)q0 << 40, b = (uint64_t)q1 << 8, s1 = a + b, s2 = s1 + (uint64_t)c8;
+    lo = s2;
+    hi = ((uint32_t)(q0 >> 24) + (uint32_t)(q1 >> 56) + (uint32_t)(c8 >> 63) + (uint32_t)(s1 < a) +
+          (uint32_t)(s2 < s1)) & 255u;
+}
+
 // ---------------------------------------------------------------------------
 // The contraction.  A block of WR x WK waves computes a (32 WR) x (32 WK)
 // tile, one 32 x 32 block per wave with nine accumulators.  Per k step the
@@ -328,39 +349,63 @@ __global__ __launch_bounds__(64 * WR * WK, OCC) void k_contract_frac(const v4i *
         body(Tail{}, g, fb0, fb1);
         if (g + 1 < G) body(Tail{}, g + 1, fb1, fb0);
     }
-    // epilogue: lane (row0 + 8 (i / 4) + i mod 4, key) for i < 16
+    // epilogue: lane (row0 + 8 (i / 4) + i mod 4, key) for i < 16.  Every
+    // output's key value is stored unless it is near a rounding change; those
+    // are marked in amb (bit i; bit 16 + i: near a half-integer) and listed
+    // afterwards by the waves that have any (rare), so the common path has no
+    // atomics and no branch per output.
+    //
+    // In 64-bit words: V = hi 2^64 + lo (hi < 2^8), p V = T_hi 2^64 + T_lo
+    // (T_hi < 2^24), the key value floor((p V + 2^71) / 2^72) =
+    // (T_hi + 128) >> 8.  The wrap test is exact (M < 2^64): V < M is hi = 0
+    // and lo < M, V > 2^72 - M is hi = 255 and 2^64 - lo < M.  The half-integer
+    // test compares the top 40 bits of p V mod 2^72 (units of 2^32) with the
+    // margin rounded outwards: p V mod 2^72 - 2^71 = 2^32 D + r with
+    // D = (T_hi mod 2^8 - 128) 2^32 + (T_lo >> 32) and 0 <= r < 2^32, so
+    // |p V mod 2^72 - 2^71| <= p M implies |D| <= floor(p M / 2^32) + 2.  The
+    // pairs this wider test adds lie within 2^34 of the exact window (valid
+    // ciphertexts put p V far from 2^71), and are settled exactly like the
+    // others.
     const int key = tk * (32 * WK) + wk * 32 + (lane & 31);
     if (key >= S) return;
     const int row0 = tr * (32 * WR) + wr * 32 + (lane >> 5) * 4;
     const uint64_t mv = out.margin[key];
-    const u128 mask72 = ((u128)1 << 72) - 1, half = (u128)1 << 71, one = (u128)1 << 72;
-    const u128 pm = (u128)IPFE_P * mv;
+    const long long pmu = (long long)(((u128)IPFE_P * mv) >> 32);
+    uint32_t amb = 0;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int row = row0 + (i >> 2) * 8 + (i & 3);
-        if (row >= R) continue;
-        u128 V = 0;
+        uint64_t lo;
+        uint32_t hi;
+        frac_value(acc, i, lo, hi);
+        const uint64_t t = (uint64_t)IPFE_P * (uint32_t)lo;                 // p lo_0
+        const uint64_t u = (uint64_t)IPFE_P * (uint32_t)(lo >> 32) + (t >> 32);  // p lo_1 + carry
+        const uint32_t thi = IPFE_P * hi + (uint32_t)(u >> 32);
+        const uint32_t tlo_hi = (uint32_t)u;  // bits 32..63 of T_lo
+        const long long D = ((long long)(thi & 255u) - 128) * 4294967296ll + tlo_hi;
+        const bool halfway = D <= pmu + 2 && D >= -pmu - 2;
+        const bool wrap = (hi == 0u && lo < mv) || (hi == 255u && lo != 0 && 0 - lo < mv);
+        const bool near = wrap || halfway;
+        if (row < R && !near) out.kval[(size_t)row * S + key] = (thi + 128u) >> 8;
+        amb |= (row < R && near ? 1u << i : 0u) | (row < R && halfway ? 1u << (16 + i) : 0u);
+    }
+    if (__ballot(amb != 0) == 0) return;
 #pragma unroll
-        for (int c = 0; c < FRAC_ACC; ++c) V += (u128)(__int128)acc[c][i] << (8 * (FRAC_ACC - 1 - c));
-        V &= mask72;
-        const u128 pv = (u128)IPFE_P * V;
-        const u128 fr = pv & mask72;
-        const bool wrap = V < mv || V > one - mv;
-        const bool halfway = fr + pm >= half && fr <= half + pm;
-        if (!wrap && !halfway) {
-            out.kval[(size_t)row * S + key] = (uint32_t)((pv + half) >> 72);
-        } else {
-            // bit 31 of the key word: near a half-integer (else only the wrap);
-            // then V's 72 bits for the refinement
-            const uint32_t n = atomicAdd(out.amb_count, 1u);
-            if ((int)n < out.amb_cap) {
-                uint32_t *e = out.amb + (size_t)FRAC_AMB_WORDS * n;
-                e[0] = (uint32_t)row;
-                e[1] = (uint32_t)key | (halfway ? 0x80000000u : 0u);
-                e[2] = (uint32_t)V;
-                e[3] = (uint32_t)(V >> 32);
-                e[4] = (uint32_t)(V >> 64);
-            }
+    for (int i = 0; i < 16; ++i) {
+        if (!((amb >> i) & 1u)) continue;
+        // bit 31 of the key word: near a half-integer (else only the wrap);
+        // then V's 72 bits for the refinement
+        uint64_t lo;
+        uint32_t hi;
+        frac_value(acc, i, lo, hi);
+        const uint32_t n = atomicAdd(out.amb_count, 1u);
+        if ((int)n < out.amb_cap) {
+            uint32_t *e = out.amb + (size_t)FRAC_AMB_WORDS * n;
+            e[0] = (uint32_t)(row0 + (i >> 2) * 8 + (i & 3));
+            e[1] = (uint32_t)key | (((amb >> (16 + i)) & 1u) ? 0x80000000u : 0u);
+            e[2] = (uint32_t)lo;
+            e[3] = (uint32_t)(lo >> 32);
+            e[4] = hi;
         }
     }
 }

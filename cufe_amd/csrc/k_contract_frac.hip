@@ -22,8 +22,8 @@
 // values and no residue planes.  Products of equal weight share an accumulator
 // C_c, c = d - e - 1 in 0..8: |C_c| <= 2 K 2^14 < 2^31 for K <= 2^15.
 //
-// The epilogue forms V = sum_c C_c 2^(8 (8 - c)) mod 2^72 in 128-bit integers,
-// so V / 2^72 = sum_k alpha~[k] B[k] mod 1 exactly, and the true x / Q lies
+// The epilogue forms V = sum_c C_c 2^(8 (8 - c)) mod 2^72 exactly (as two
+// 64-bit words), so V / 2^72 = sum_k alpha~[k] B[k] mod 1, and the true x / Q lies
 // within M_v 2^-72 of it, M_v = ceil(s / 2) + ceil(s / 2^16) + 1 with
 // s = sum_k |B_v[k]| (the rounding of alpha~, below).  The key
 // value floor((p V + 2^71) / 2^72) is round(p x / Q) unless that interval

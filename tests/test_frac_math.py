@@ -7,7 +7,9 @@ signed digits per column lies within the kernel's margin M_v of
 dropped digit pair (1, 1) contributes only multiples of 2^72.  The exact
 kernel's refinement V88 = 2^16 V + sum_k D[k] B[k] mod 2^88 (D: the 16 bits
 below the digits) lies within M88_v of 2^88 x / Q and decides every pair the
-same way at 88 bits."""
+same way at 88 bits.  The epilogue's 64-bit word arithmetic equals the
+128-bit one (its half-integer test a superset) on random and boundary
+accumulators."""
 import hashlib
 from fractions import Fraction
 
@@ -123,3 +125,79 @@ def test_fraction_sum_within_margin_and_rounds_like_the_oracle():
             # about 2^-66 of a rounding change
             assert decide(V88, M88, 88) == ref
     assert checked >= 10 and wraps >= 3
+
+
+def words_of_v(acc):
+    """The kernel's frac_value: V = sum_c C_c 2^(8 (8 - c)) mod 2^72 as two
+    words (lo 64 bits, hi 8 bits), from pairwise 64-bit merges and carries."""
+    m64 = (1 << 64) - 1
+    p0, p1 = acc[0] * 256 + acc[1], acc[2] * 256 + acc[3]
+    p2, p3 = acc[4] * 256 + acc[5], acc[6] * 256 + acc[7]
+    q0, q1, c8 = p0 * 65536 + p1, p2 * 65536 + p3, acc[8]
+    assert abs(q0) < 1 << 57 and abs(q1) < 1 << 57
+    a, b = (q0 << 40) & m64, (q1 << 8) & m64
+    s1 = (a + b) & m64
+    s2 = (s1 + (c8 & m64)) & m64
+    hi = ((q0 >> 24) + (q1 >> 56) + (c8 >> 63) + (s1 < a) + (s2 < s1)) & 255
+    return s2, hi
+
+
+def decide_words(lo, hi, m):
+    """The kernel's epilogue tests on 64-bit words: (near a half-integer, near
+    the wrap, key value)."""
+    t = P * (lo & 0xFFFFFFFF)
+    u = P * (lo >> 32) + (t >> 32)
+    thi = (P * hi + (u >> 32)) & 0xFFFFFFFF
+    d = ((thi & 255) - 128) * (1 << 32) + (u & 0xFFFFFFFF)
+    pmu = (P * m) >> 32
+    halfway = -pmu - 2 <= d <= pmu + 2
+    wrap = (hi == 0 and lo < m) or (hi == 255 and lo != 0 and (1 << 64) - lo < m)
+    return halfway, wrap, (thi + 128) >> 8
+
+
+def test_epilogue_words_against_exact_integers():
+    """The 64-bit epilogue against 128-bit integer arithmetic: V's words, the
+    key value wherever no test fires, the wrap test exactly, and a
+    half-integer test that fires whenever the exact one does.  Accumulators
+    are random or built so that V sits on and around 0, 2^72 and the
+    half-integers of p V / 2^72, for margins from 1 to the 2^62 cap."""
+    rng = np.random.default_rng(8)
+    one = 1 << 72
+
+    def balanced(v):
+        # nine accumulators summing to v mod 2^72, then 2^8 r moved between
+        # neighbours so that they are large (as after 4,224 products)
+        c = [0] * 9
+        for k in range(8, 0, -1):
+            c[k] = ((v & 255) + 128) % 256 - 128
+            v = (v - c[k]) >> 8
+        c[0] = ((v & 255) + 128) % 256 - 128
+        for k in range(8, 0, -1):
+            r = int(rng.integers(-(1 << 22), 1 << 22))
+            c[k] += 256 * r
+            c[k - 1] -= r
+        return c
+
+    cases = [[int(x) for x in rng.integers(-(1 << 31) + 1, 1 << 31, 9)] for _ in range(3000)]
+    for m in (1, 1 << 21, 1 << 33, 1 << 50, 1 << 62):
+        for base in (0, one):
+            for d in (-m - 1, -m, -m + 1, -2, -1, 0, 1, 2, m - 1, m, m + 1, 1 << 32, -(1 << 32)):
+                cases.append((balanced((base + d) % one), m))
+        for k in range(0, P, 4999):
+            v0 = ((2 * k + 1) << 71) // P  # p V / 2^72 close to k + 1/2
+            for d in (-3, -1, 0, 1, 3, m, -m, 1 << 32):
+                cases.append((balanced((v0 + d) % one), m))
+    margins = (1, 7, 1 << 21, 1 << 40, 1 << 62)
+    for n, case in enumerate(cases):
+        acc, m = case if isinstance(case, tuple) else (case, margins[n % len(margins)])
+        v = sum(c << (8 * (8 - k)) for k, c in enumerate(acc)) % one
+        lo, hi = words_of_v(acc)
+        assert lo + (hi << 64) == v
+        halfway, wrap, kv = decide_words(lo, hi, m)
+        fr = P * v % one
+        e_wrap = v < m or v > one - m
+        e_half = fr + P * m >= 1 << 71 and fr <= (1 << 71) + P * m
+        assert wrap == e_wrap, (acc, m)
+        assert halfway or not e_half, (acc, m)
+        if not (halfway or wrap):
+            assert kv == (P * v + (1 << 71)) >> 72

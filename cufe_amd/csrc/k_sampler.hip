@@ -508,24 +508,35 @@ __global__ void k_gauss_scatter(const GaussStream *__restrict__ streams, int nst
                                 const uint8_t *__restrict__ cnt, const uint64_t *__restrict__ vals,
                                 const int *__restrict__ pos)
 {
+    // The 8 chunks of a wave (64 consecutive t) nearly always belong to one
+    // stream: it is found once per wave from the wave's first chunk, with
+    // wave-uniform loads, and per lane only in a wave that crosses a stream.
+    auto one = [&](const GaussStream *S, long long gc, int slot) {
+        const int info = pos[gc];
+        if (info < 0) return;
+        const int n = cnt[gc];
+        const int call = info >> 13, off = info & 0x1fff;
+        if (slot >= n || off + slot >= S->slen) return;
+        const uint64_t v = vals[gc * 8 + slot];
+        const uint64_t z = v & 0x7fffffffffffffffull;
+        const bool neg = (v >> 63) != 0;
+        // output residues (src/gauss.c:773-779)
+        uint32_t *out = ((call == 0) ? S->out0 : S->out1 + (long long)(call - 1) * S->out_stride) + off + slot;
+        for (int s = 0; s < S->nmod; ++s) out[(long long)s * S->slen] = stream_residue(S, s, z, neg);
+    };
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nchunks_total * 8;
          t += (long long)gridDim.x * blockDim.x) {
-    const long long gc = t >> 3;
-    const int slot = (int)(t & 7);
-    const int info = pos[gc];
-    if (info < 0) continue;
-    const int n = cnt[gc];
-    const int call = info >> 13, off = info & 0x1fff;
-    if (slot >= n) continue;
-    const int si = find_stream(chunk_prefix, nstreams, gc);
-    const GaussStream *S = streams + si;
-    if (off + slot >= S->slen) continue;
-    const uint64_t v = vals[gc * 8 + slot];
-    const uint64_t z = v & 0x7fffffffffffffffull;
-    const bool neg = (v >> 63) != 0;
-    // output residues (src/gauss.c:773-779)
-    uint32_t *out = ((call == 0) ? S->out0 : S->out1 + (long long)(call - 1) * S->out_stride) + off + slot;
-    for (int s = 0; s < S->nmod; ++s) out[(long long)s * S->slen] = stream_residue(S, s, z, neg);
+        const long long gc = t >> 3;
+        const int slot = (int)(t & 7);
+        // the lowest t of the wave is its first active lane's
+        const long long gc0 = ((long long)__builtin_amdgcn_readfirstlane((int)(gc >> 32)) << 32) |
+                              (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)gc);
+        const int sw = __builtin_amdgcn_readfirstlane(find_stream(chunk_prefix, nstreams, gc0));
+        const long long end = sw + 1 < nstreams ? chunk_prefix[sw + 1] : nchunks_total;
+        if (__all(gc < end))
+            one(streams + sw, gc, slot);
+        else
+            one(streams + find_stream(chunk_prefix, nstreams, gc), gc, slot);
     }
 }
 

@@ -421,10 +421,7 @@ constexpr int kSplitMaxRows = 256;
 constexpr int kAmbCap = 1 << 20;
 constexpr size_t kSplitMaxPairs = (size_t)1 << 19;
 
-bool split_k_range(const Model &m, int R)
-{
-    return m.i8 && m.fused && m.nb == 2 && R <= kSplitMaxRows && (size_t)R * m.S <= kSplitMaxPairs;
-}
+bool large_range(const Model &m, int R) { return R > kSplitMaxRows || (size_t)R * m.S > kSplitMaxPairs; }
 
 size_t Scratch::split_bytes(int rows, int S)
 {

@@ -115,10 +115,12 @@ struct PredictTimers {
 int predict_timers_collect(PredictTimers *tm);
 int predict_rows(const Model &m, const RowStore &rs, int r0, int R, Scratch &sc, int scratch_rows, int mode,
                  double *labels_host, uint32_t *kval_host, PredictTimers *tm, hipStream_t st);
-// Whether predict_rows runs R rows of m (from a 32-row boundary) as one
-// split-K launch when its scratch allows it (Scratch::split_ok): small ranges
-// of fused two-digit keys.  Larger ranges run the per-limb launches.
-bool split_k_range(const Model &m, int R);
+// Whether R rows of m are a range too large for the split-K launch (more
+// than 256 rows or 2^19 (row, key) pairs): predict_rows runs it as full-GPU
+// per-limb or fraction-form launches.  Smaller ranges take the split-K launch
+// when the model's keys allow it (fused, two digits) and the scratch does
+// (Scratch::split_ok), else small per-limb launches.
+bool large_range(const Model &m, int R);
 
 // Encrypts n rows x [n][l] (row b with nonce nonce0 + b) into store rows
 // [r0, r0+n), against master keys mpk [l_key+1][3][N] on the device.

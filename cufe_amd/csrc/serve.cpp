@@ -88,7 +88,7 @@ struct Slot {
     std::vector<Group> groups;
     bool busy = false;
     int batch_rows = 0;  // rows of the batch in flight (its aligned layout)
-    bool big = false;    // a group of the batch in flight runs the per-limb launches
+    bool big = false;    // a group of the batch in flight is a large range (full-GPU launches)
     size_t batch_bytes = 0;  // HBM its rows occupy (rows x bytes per row)
     bool failed = false;
     std::string err;
@@ -214,14 +214,15 @@ int launch_batch(cufe_server *s, Slot &sl)
         i = j;
     }
     // small groups take the low-latency split-K launch unless another slot
-    // runs a group on the per-limb launches (under load the split launch
-    // would slow that contraction more than it gains).  A batch of at most
-    // kBatchRows rows never does, whatever its aligned layout (up to 31
-    // padding rows per model): only a request part larger than that.
+    // runs a large range (full-GPU launches, which under load the split launch
+    // would slow more than it gains).  A batch of at most kBatchRows rows has
+    // none, whatever its aligned layout (up to 31 padding rows per model) and
+    // whether or not its models can use the split-K launch: only a request
+    // part larger than that does.
     sl.batch_rows = row;
     sl.batch_bytes = (size_t)row * s->row_bytes;
     sl.big = false;
-    for (const Group &gr : sl.groups) sl.big = sl.big || !split_k_range(*gr.m, gr.R);
+    for (const Group &gr : sl.groups) sl.big = sl.big || large_range(*gr.m, gr.R);
     bool large_other = false;
     for (const auto &o : s->slots)
         if (o.get() != &sl && o->busy && o->big) large_other = true;
